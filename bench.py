@@ -1,0 +1,197 @@
+"""Benchmark of the fused fold pipeline on one MI355X.
+
+Workload: causal attention written as a fold pipeline (scores = SUM of
+q*k grouped by h, tq, tk with tq >= tk, softmax-weighted sum of v), 8 heads,
+T = 4096, d = 64, q and k ~ N(0,1)/8, v and the upstream gradient ~ N(0,1),
+synthetic data from a seed, inputs resident in HBM.  A step is one forward
+plus backward pass (dq, dk, dv of SUM(out * g)) in the mixed form: BF16x3
+MFMA products with FP32 accumulation.  The same step in plain FP32 (FP32
+MFMA) is timed beside it, and both are checked against the FP64 GPU form.
+
+value = algorithmic GFLOP/s of the mixed step: 14 * H * d * P FLOP for P
+allowed (query, key) pairs per head (forward 2 products, backward 5).
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+H, T, D = 8, 4096, 64
+SCALE = 1.0 / 8.0
+PEAK_BF16 = 2.5e15       # MI355X dense BF16 MFMA, FLOP/s
+PEAK_FP32 = 157.3e12
+
+
+def pairs_per_head(t: int) -> int:
+    return t * (t + 1) // 2
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from einfold_amd import _lib, data, device
+    from einfold_amd._lib import lib, check
+
+    dev = torch.device("cuda:0")
+    check(lib().ef_init(0))
+    q64, k64, v64 = data.attention_tables(H, T, D, seed=3)
+    g64 = data.normal(3, 4, H * T * D).reshape(H, T, D)
+    to = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x)).to(dt).to(dev)
+    q32, k32, v32, g32 = (to(x, torch.float32) for x in (q64, k64, v64, g64))
+    qd, kd, vd, gd = (to(x, torch.float64) for x in (q64, k64, v64, g64))
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def make_desc(precision):
+        desc = _lib.CAttentionDesc()
+        desc.batches, desc.queries, desc.keys, desc.features, desc.values = H, T, T, D, D
+        desc.scale = SCALE
+        desc.allowed = None
+        desc.causal, desc.causal_offset, desc.window = 1, 0, 0
+        desc.precision = _lib.PRECISION[precision]
+        return desc
+
+    def buffers(qx, dtype):
+        return dict(out=torch.empty_like(qx), m=torch.empty((H, T), dtype=torch.float32, device=dev),
+                    l=torch.empty((H, T), dtype=torch.float32, device=dev),
+                    dq=torch.empty_like(qx), dk=torch.empty_like(qx), dv=torch.empty_like(qx),
+                    scratch=torch.empty(H * T * 4, dtype=torch.uint8, device=dev))
+
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    def step(desc, q, k, v, g, bf, kv_events=None):
+        L = lib()
+        check(L.ef_attention_fwd(ctypes.byref(desc), P(q), P(k), P(v), P(bf["out"]), P(bf["m"]),
+                                 P(bf["l"]), None, stream))
+        check(L.ef_attention_bwd_prepare(ctypes.byref(desc), P(bf["out"]), P(g), P(bf["scratch"]), stream))
+        if kv_events:
+            kv_events[0].record()
+        check(L.ef_attention_bwd_kv(ctypes.byref(desc), P(q), P(k), P(v), P(g), P(bf["m"]), P(bf["l"]),
+                                    P(bf["scratch"]), P(bf["dk"]), P(bf["dv"]), stream))
+        if kv_events:
+            kv_events[1].record()
+        check(L.ef_attention_bwd_q(ctypes.byref(desc), P(q), P(k), P(v), P(g), P(bf["m"]), P(bf["l"]),
+                                   P(bf["scratch"]), P(bf["dq"]), stream))
+
+    def timed(desc, q, k, v, g, bf, steps, warmup, kv_time=False):
+        for _ in range(warmup):
+            step(desc, q, k, v, g, bf)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)] if kv_time else [None] * steps
+        t0 = time.perf_counter()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for i in range(steps):
+            step(desc, q, k, v, g, bf, evs[i])
+        stop.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = start.elapsed_time(stop) / steps
+        kv_ms = float(np.mean([a.elapsed_time(b) for a, b in evs])) if kv_time else None
+        return ms, kv_ms, wall
+
+    # FP64 reference (the GPU's FP64 form), once.
+    ref = buffers(qd, torch.float64)
+    step(make_desc("fp64"), qd, kd, vd, gd, ref)
+    torch.cuda.synchronize()
+
+    def rel(x, r):
+        return float(torch.linalg.vector_norm(x.double() - r) / torch.linalg.vector_norm(r))
+
+    def errors(bf):
+        return {name: rel(bf[name], ref[name]) for name in ("out", "dq", "dk", "dv")}
+
+    mixed = buffers(q32, torch.float32)
+    mixed_desc = make_desc("bf16x3")
+    ms, kv_ms, wall = timed(mixed_desc, q32, k32, v32, g32, mixed, args.steps, args.warmup, kv_time=True)
+    mixed_err = errors(mixed)
+    plain = buffers(q32, torch.float32)
+    plain_desc = make_desc("fp32")
+    fp32_ms, fp32_kv_ms, _ = timed(plain_desc, q32, k32, v32, g32, plain, args.steps, args.warmup, kv_time=True)
+    plain_err = errors(plain)
+
+    pairs = H * pairs_per_head(T)
+    flop_step = 14 * D * pairs
+    flop_kv = 8 * D * pairs            # S, dP, dV, dK products of the dK/dV kernel
+    value = flop_step / (ms * 1e-3) / 1e9
+    achieved = flop_kv / (kv_ms * 1e-3)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+
+    line = {
+        "metric": "fold GFLOP/s + attention fwd+bwd ms: mixed vs plain FP32, 1 MI355X",
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3 (fp32 accumulate)",
+        "data": "synthetic (SplitMix64 seed 3), inputs resident in HBM",
+        "config": {"workload": "causal attention fold pipeline fwd+bwd, 8 heads x T=4096 x d=64",
+                   "heads": H, "seq_len": T, "head_dim": D, "mask": "causal", "parallelism": "none"},
+        "attention_fwd_bwd_ms": {"mixed_bf16x3": round(ms, 4), "plain_fp32": round(fp32_ms, 4),
+                                 "speedup": round(fp32_ms / ms, 3)},
+        "rel_error_vs_fp64": {"mixed_bf16x3": mixed_err, "plain_fp32": plain_err},
+        "roofline": {"kernel": "attn_bwd_dkdv_kernel<bf16x3, 64>", "bound": "mfma",
+                     "achieved": round(achieved / 1e12, 2), "peak": PEAK_BF16 / 1e12,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4),
+                     "issued_frac": round(3 * achieved / PEAK_BF16, 4),
+                     "avg_ms": round(kv_ms, 4), "traffic": None,
+                     "fp32_kernel": {"avg_ms": round(fp32_kv_ms, 4),
+                                     "frac_of_fp32_peak": round(flop_kv / (fp32_kv_ms * 1e-3) / PEAK_FP32, 4)}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline():
+    """The oracle's restatement of einfold's fused CPU kernel (fused.rs
+    tile loop, FP64, one thread) on one head of the workload, forward only
+    (the reference has no backward)."""
+    import oracle
+    from oracle import reference as R
+    from einfold_amd import data, ir
+    q, k, v = data.attention_tables(1, T, D, seed=3)
+    lo = np.zeros(T, np.int64)
+    hi = np.arange(1, T + 1, dtype=np.int64)
+    p = ir.Pipeline(producer=ir.Fold([], [], []), weight=ir.Exp(ir.Scale(ir.ScaleKind.TIMES, SCALE)),
+                    dimensions=[], values=ir.Operand((), 0), outputs=[])
+    kt = np.ascontiguousarray(k[0].T)
+    t0 = time.perf_counter()
+    R.fused_core(p, T, D, T, D, q[0], kt, v[0], lo, hi)
+    dt = time.perf_counter() - t0
+    flop = 4 * D * pairs_per_head(T)
+    return {"value": round(flop / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "seconds": round(dt, 3),
+            "sample": "1 of 8 heads, forward only, T=4096, d=64, causal, FP64, the oracle's C "
+                      "restatement of fused.rs's tile loop on 1 thread"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+// Wave-level 16x16 MFMA fragments for gfx950, in one logical layout for
+// every precision.
+//
+// A 16x16 output block C[m][n] += sum_k A[m][k] B[k][n] is computed over
+// K in chunks of 32.  In each chunk, lane l (group g = l / 16) holds 8
+// logical k-values of its row m = l % 16 of A and of its column n = l % 16
+// of B: logical k = g*8 + j, j = 0..7.  The physical k a logical k stands
+// for is the caller's choice, as long as A and B agree:
+//   * contiguous:  physical k = g*8 + j            (operands read from memory)
+//   * cperm:       physical k = g*4 + j (j < 4), 16 + g*4 + (j-4) (j >= 4)
+//                  (an operand taken from two stacked C blocks in registers)
+// The C block's layout is the same for every precision: lane l holds
+// C[g*4 + i][l % 16], i = 0..3.
+//
+// Precisions: FP32 issues eight v_mfma_f32_16x16x4_f32 per chunk (step t
+// takes logical k = g*8 + t from group g); FP64 likewise with
+// v_mfma_f64_16x16x4_f64; BF16 one v_mfma_f32_16x16x32_bf16; BF16X3 three
+// of them on split operands x = hi + lo (hi = bf16(x), lo = bf16(x - hi)),
+// hi*hi + hi*lo + lo*hi, recovering about 16 more bits than one BF16
+// product.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ef {
+
+enum : int { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2, PREC_BF16 = 3 };
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int P> struct Frag;
+template <> struct Frag<PREC_F32> { float v[8]; };
+template <> struct Frag<PREC_F64> { double v[8]; };
+template <> struct Frag<PREC_BF16> { bf16x8 hi; };
+template <> struct Frag<PREC_BF16X3> { bf16x8 hi, lo; };
+
+template <int P> struct Acc { typedef f32x4 type; typedef float scalar; };
+template <> struct Acc<PREC_F64> { typedef f64x4 type; typedef double scalar; };
+
+// Element type of a precision's tiles in LDS, and how many planes (hi, lo).
+template <int P> struct Elem { typedef float T; static constexpr int planes = 1; };
+template <> struct Elem<PREC_F64> { typedef double T; static constexpr int planes = 1; };
+template <> struct Elem<PREC_BF16> { typedef __bf16 T; static constexpr int planes = 1; };
+template <> struct Elem<PREC_BF16X3> { typedef __bf16 T; static constexpr int planes = 2; };
+
+template <int P> struct Input { typedef float T; };
+template <> struct Input<PREC_F64> { typedef double T; };
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+template <int P> __device__ __forceinline__ void mma(typename Acc<P>::type &c, const Frag<P> &a, const Frag<P> &b);
+
+template <> __device__ __forceinline__ void mma<PREC_F32>(f32x4 &c, const Frag<PREC_F32> &a, const Frag<PREC_F32> &b) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v[t], b.v[t], c, 0, 0, 0);
+}
+template <> __device__ __forceinline__ void mma<PREC_F64>(f64x4 &c, const Frag<PREC_F64> &a, const Frag<PREC_F64> &b) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a.v[t], b.v[t], c, 0, 0, 0);
+}
+template <> __device__ __forceinline__ void mma<PREC_BF16>(f32x4 &c, const Frag<PREC_BF16> &a, const Frag<PREC_BF16> &b) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+}
+template <> __device__ __forceinline__ void mma<PREC_BF16X3>(f32x4 &c, const Frag<PREC_BF16X3> &a, const Frag<PREC_BF16X3> &b) {
+    // Small products first, so the large one is added to their sum.
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+}
+
+// A fragment from 8 values in registers (split for BF16 precisions).
+template <int P, class T> __device__ __forceinline__ void make_frag(Frag<P> &f, const T (&x)[8]) {
+    if constexpr (P == PREC_F32) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) f.v[j] = (float)x[j];
+    } else if constexpr (P == PREC_F64) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) f.v[j] = (double)x[j];
+    } else if constexpr (P == PREC_BF16) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) f.hi[j] = (__bf16)(float)x[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            __bf16 h, l;
+            split_bf16((float)x[j], h, l);
+            f.hi[j] = h;
+            f.lo[j] = l;
+        }
+    }
+}
+
+// Stores one value into a tile of `plane` elements per plane.
+template <int P, class T>
+__device__ __forceinline__ void tile_store(typename Elem<P>::T *base, int plane, int idx, T x) {
+    if constexpr (P == PREC_BF16X3) {
+        __bf16 h, l;
+        split_bf16((float)x, h, l);
+        base[idx] = h;
+        base[plane + idx] = l;
+    } else {
+        base[idx] = (typename Elem<P>::T)x;
+    }
+}
+
+// Loads logical k = 0..7 from base[idx .. idx+7] (idx 8-element aligned).
+template <int P>
+__device__ __forceinline__ void tile_load8(Frag<P> &f, const typename Elem<P>::T *base, int plane, int idx) {
+    if constexpr (P == PREC_F32) {
+        const f32x4 a = *reinterpret_cast<const f32x4 *>(base + idx);
+        const f32x4 b = *reinterpret_cast<const f32x4 *>(base + idx + 4);
+#pragma unroll
+        for (int j = 0; j < 4; j++) { f.v[j] = a[j]; f.v[j + 4] = b[j]; }
+    } else if constexpr (P == PREC_F64) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) f.v[j] = base[idx + j];
+    } else if constexpr (P == PREC_BF16) {
+        f.hi = *reinterpret_cast<const bf16x8 *>(base + idx);
+    } else {
+        f.hi = *reinterpret_cast<const bf16x8 *>(base + idx);
+        f.lo = *reinterpret_cast<const bf16x8 *>(base + plane + idx);
+    }
+}
+
+// Loads logical k = 0..3 from base[i0 .. i0+3] and 4..7 from base[i1 .. i1+3]
+// (the cperm pattern; each index 4-element aligned).
+template <int P>
+__device__ __forceinline__ void tile_load4x2(Frag<P> &f, const typename Elem<P>::T *base, int plane, int i0, int i1) {
+    if constexpr (P == PREC_F32) {
+        const f32x4 a = *reinterpret_cast<const f32x4 *>(base + i0);
+        const f32x4 b = *reinterpret_cast<const f32x4 *>(base + i1);
+#pragma unroll
+        for (int j = 0; j < 4; j++) { f.v[j] = a[j]; f.v[j + 4] = b[j]; }
+    } else if constexpr (P == PREC_F64) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) { f.v[j] = base[i0 + j]; f.v[j + 4] = base[i1 + j]; }
+    } else {
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(base + i0);
+        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(base + i1);
+#pragma unroll
+        for (int j = 0; j < 4; j++) { f.hi[j] = a[j]; f.hi[j + 4] = b[j]; }
+        if constexpr (P == PREC_BF16X3) {
+            const bf16x4 c = *reinterpret_cast<const bf16x4 *>(base + plane + i0);
+            const bf16x4 d = *reinterpret_cast<const bf16x4 *>(base + plane + i1);
+#pragma unroll
+            for (int j = 0; j < 4; j++) { f.lo[j] = c[j]; f.lo[j + 4] = d[j]; }
+        }
+    }
+}
+
+template <class T> __device__ __forceinline__ T wave_max(T x) {
+    x = x > __shfl_xor(x, 16) ? x : __shfl_xor(x, 16);
+    x = x > __shfl_xor(x, 32) ? x : __shfl_xor(x, 32);
+    return x;
+}
+
+// Sum over the four lane groups that share a column (l % 16), in a fixed
+// order: (g0 + g1) + (g2 + g3) on every lane.
+template <class T> __device__ __forceinline__ T group_sum(T x) {
+    x = x + __shfl_xor(x, 16);
+    x = x + __shfl_xor(x, 32);
+    return x;
+}
+
+}  // namespace ef

@@ -1,0 +1,330 @@
+"""Pins the oracle (CPU only): against the reference's own known answers,
+closed forms, an independent nested-loop SQL evaluation, and exact
+metamorphic relations, chosen so that a dropped term, a wrong index or a
+transposed operand fails one of them."""
+import math
+
+import numpy as np
+import pyarrow as pa
+import pytest
+
+import oracle
+from oracle import reference as R
+from einfold_amd import data, ir
+
+from sqlsem import fold_rows, within_bound
+
+EQ = ir.KeyEquality.EQUAL
+
+
+# ------------------------------------------------------------ generator
+def test_splitmix64_known_answers():
+    """crates/einfold-testkit/src/rng.rs:65-78."""
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    assert [int(x) for x in oracle.splitmix_next(0, 0, 4)] == want
+    assert [int(x) for x in data.next_u64(0, 0, 4)] == want
+
+
+def test_vectorized_generator_matches_the_oracles():
+    for seed, salt in [(0x5EED, 1), (7, 0), (123456789, 99)]:
+        assert np.array_equal(oracle.splitmix_next(seed, salt, 1000), data.next_u64(seed, salt, 1000))
+    # A different salt, a different stream (rng.rs:80-89).
+    assert not np.array_equal(data.next_u64(0x5EED, 1, 8), data.next_u64(0x5EED, 2, 8))
+
+
+# ------------------------------------------------------------ dense
+def _random_dense(rng, nx, nz, ny, nw=1, holes=True):
+    X = nx * nw
+    a = rng.standard_normal((X, nz)) * 10.0 ** rng.integers(-3, 4, (X, nz))
+    held = (rng.choice([0, 1, 2, 2, 2], size=(X, nz)) if holes else np.full((X, nz), 2)).astype(np.uint8)
+    b = rng.standard_normal((nw, nz, ny))
+    return a, held, b
+
+
+@pytest.mark.parametrize("nx,nz,ny,nw", [(17, 33, 9, 1), (64, 80, 70, 2), (3, 4097, 5, 1), (1, 9000, 2, 2)])
+def test_dense_accumulate_within_summation_bound(nx, nz, ny, nw):
+    """numpy float64 einsum within 2(n-1)u sum|terms| (kernels_vs_sql.rs:19-23)."""
+    rng = np.random.default_rng(nx + nz + ny)
+    a, held, b = _random_dense(rng, nx, nz, ny, nw)
+    got = oracle.dense_accumulate(a, held, b, nx * nw, nz, ny, nx).reshape(nx * nw, ny)
+    av = np.where(held == 2, a, 0.0)
+    w = np.repeat(np.arange(nw), nx)
+    want = np.einsum("xz,xzy->xy", av, b[w])
+    mag = np.einsum("xz,xzy->xy", np.abs(av), np.abs(b[w]))
+    u = np.finfo(float).eps / 2
+    assert np.all(np.abs(got - want) <= 2 * max(nz - 1, 1) * u * mag + 1e-300)
+
+
+def test_dense_accumulate_exact_metamorphic():
+    """[scale]: doubling a doubles, negating negates, bit for bit
+    (dense_metamorphic.rs:33-37)."""
+    rng = np.random.default_rng(3)
+    for (nx, nz, ny) in [(70, 65, 70), (5, 5000, 3)]:
+        a, held, b = _random_dense(rng, nx, nz, ny)
+        base = oracle.dense_accumulate(a, held, b, nx, nz, ny, nx)
+        assert np.array_equal(oracle.dense_accumulate(2 * a, held, b, nx, nz, ny, nx), 2 * base)
+        assert np.array_equal(oracle.dense_accumulate(-a, held, b, nx, nz, ny, nx), -base)
+        # A transposed operand or dropped cell changes the result.
+        held2 = held.copy()
+        held2[0, np.argmax(held[0] == 2)] = 0
+        assert not np.array_equal(oracle.dense_accumulate(a, held2, b, nx, nz, ny, nx), base)
+
+
+def test_dense_accumulate_skips_non_valued_cells_exactly():
+    """A NULL or absent cell of a meeting an infinite b adds nothing
+    (dense.rs:716-724): the result stays finite; a valued one gives NaN/inf."""
+    a = np.array([[1.0, 2.0]])
+    b = np.array([[[3.0]], [[np.inf]]])
+    held = np.array([[2, 1]], np.uint8)
+    assert oracle.dense_accumulate(a, held, b, 1, 2, 1, 1)[0] == 3.0
+    held = np.array([[2, 2]], np.uint8)
+    assert oracle.dense_accumulate(a, held, b, 1, 2, 1, 1)[0] == np.inf
+
+
+def _grid_fold():
+    return ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                              ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)],
+                   [ir.Output(0), ir.Output(2)])
+
+
+def test_fold_dense_full_grid_against_numpy():
+    """The dense positional fold's workload: every cell of a 256^2 grid;
+    keys exact and in (i, k) order, sums within the bound of A @ B."""
+    a, b = data.grid_matmul(256)
+    f = _grid_fold()
+    sch = R.schema(f, [a.schema, b.schema])
+    out = R.fold_dense(f, a.to_batches(), b.to_batches(), None, sch)
+    assert len(out) == 1 and out[0].num_rows == 65536
+    i = out[0].column(0).to_numpy()
+    k = out[0].column(1).to_numpy()
+    assert np.array_equal(i, np.repeat(np.arange(256), 256))
+    assert np.array_equal(k, np.tile(np.arange(256), 256))
+    A = a.column("v").to_numpy().reshape(256, 256)
+    B = b.column("v").to_numpy().reshape(256, 256)
+    mag = np.abs(A) @ np.abs(B)
+    got = out[0].column(2).to_numpy().reshape(256, 256)
+    assert np.all(np.abs(got - A @ B) <= 2 * 255 * np.finfo(float).eps / 2 * mag)
+
+
+def _table(rows, names, types):
+    cols = list(zip(*rows)) if rows else [[] for _ in names]
+    return pa.table({n: pa.array(list(c), type=t) for n, c, t in zip(names, cols, types)})
+
+
+def _random_sparse_case(rng, n_a=60, n_b=50, nulls=True, keyspace=6):
+    def key():
+        if nulls and rng.random() < 0.08:
+            return None
+        return int(rng.integers(0, keyspace))
+
+    def fac():
+        if nulls and rng.random() < 0.1:
+            return None
+        return float(rng.choice([-2.0, -1.0, -0.5, 0.0, -0.0, 0.25, 1.0, 3.0]))
+    a = [((key(), key()), fac()) for _ in range(n_a)]
+    b = [((key(), key()), fac()) for _ in range(n_b)]
+    return a, b
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("strict", [True, False])
+def test_discriminate_against_nested_loop_sql(seed, strict):
+    """[oracle] of dense_metamorphic.rs:26-31 for the sorting path: NULL keys
+    and factors, duplicates, bag semantics."""
+    rng = np.random.default_rng(seed)
+    a_rows, b_rows = _random_sparse_case(rng)
+    eqs = [EQ if strict else ir.KeyEquality.NOT_DISTINCT] * 3
+    f = ir.Fold(eqs, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                      ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)],
+                [ir.Output(0), ir.Output(2)])
+    ta = _table([(k[0], k[1], v) for k, v in a_rows], ["i", "j", "v"], [pa.int64(), pa.int64(), pa.float64()])
+    tb = _table([(k[0], k[1], v) for k, v in b_rows], ["j", "k", "v"], [pa.int64(), pa.int64(), pa.float64()])
+    sch = R.schema(f, [ta.schema, tb.schema])
+    algo, out = R.fold(f, ta.to_batches(), tb.to_batches(), None, sch)
+    assert algo == "discriminate"
+    want = fold_rows(a_rows, b_rows, [0, 1], [1, 2], [0, 2], {1: strict})
+    got = {}
+    t = pa.Table.from_batches(out)
+    for i, k, v in zip(t.column(0).to_pylist(), t.column(1).to_pylist(), t.column(2).to_pylist()):
+        assert (i, k) not in got
+        got[(i, k)] = v
+    assert set(got) == set(want)
+    for g, (s, mag, n) in want.items():
+        assert within_bound(got[g], s, mag, n), (g, got[g], s)
+    # Groups come in key order: NULL first, then ascending (row format).
+    keys = [(x is not None, x if x is not None else 0, y is not None, y if y is not None else 0)
+            for x, y in zip(t.column(0).to_pylist(), t.column(1).to_pylist())]
+    assert keys == sorted(keys)
+
+
+def test_ranks_closed_form_and_signed_order():
+    """Ranks are positions in numpy.unique, first rows its first occurrences
+    (discriminate.rs:280-292); Int64 keys order as signed numbers."""
+    rng = np.random.default_rng(9)
+    w = rng.integers(0, 1000, 20000).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    rank, first = oracle.ranks_of_words(w)
+    u, idx, inv = np.unique(w, return_index=True, return_inverse=True)
+    assert np.array_equal(rank, inv) and np.array_equal(first, idx)
+    s = pa.array(np.array([5, -3, 0, -3, 2 ** 40, -(2 ** 62)], dtype=np.int64))
+    r, fst = R.rank_columns([s])
+    assert list(r) == [3, 1, 2, 1, 4, 0]
+    assert list(fst) == [5, 1, 2, 0, 4]
+
+
+def test_row_format_ranks_of_composite_keys():
+    """Composite, nullable and float keys rank as Arrow's row format orders
+    them: NULL first, integers numerically, floats by total order."""
+    a = pa.array([3, None, 3, 1, None, 1], type=pa.int32())
+    f = pa.array([0.5, 1.0, -0.0, float("nan"), 1.0, -1.0], type=pa.float64())
+    r, first = R.rank_columns([a, f])
+    # Order: (NULL,1.0) rows 1 and 4 share a rank; (1,-1.0) < (1,NaN); (3,-0.0) < (3,0.5).
+    assert list(r) == [4, 0, 3, 2, 0, 1]
+    assert list(first) == [1, 5, 3, 2, 0]
+    zeros = pa.array([0.0, -0.0], type=pa.float64())
+    assert R.rank_columns([R._host_keys(zeros, True)])[0].tolist() == [0, 0]
+    assert R.rank_columns([zeros])[0].tolist() == [1, 0]
+
+
+# ------------------------------------------------------------ fused
+def _np_attention(q, k, v, scale, lo, hi):
+    s = (q @ k.T) * scale
+    j = np.arange(k.shape[0])
+    mask = (j[None, :] >= lo[:, None]) & (j[None, :] < hi[:, None])
+    s = np.where(mask, s, -np.inf)
+    m = s.max(axis=1, keepdims=True)
+    w = np.where(mask, np.exp(s - m), 0.0)
+    return (w @ v) / w.sum(axis=1, keepdims=True)
+
+
+def _p(scale=1.0, normalizer=ir.Normalizer.KEPT_GROUPS, shift=ir.Shift.KEPT_MAXIMUM):
+    return ir.Pipeline(producer=ir.Fold([], [], []),
+                       weight=ir.Exp(ir.Scale(ir.ScaleKind.TIMES, scale), shift),
+                       dimensions=[], values=ir.Operand((), 0), outputs=[], normalizer=normalizer)
+
+
+@pytest.mark.parametrize("T,window", [(64, 0), (200, 0), (200, 17), (130, 1000)])
+def test_fused_against_float64_softmax(T, window):
+    """Within 1e-9 * max|v| (pipeline_vs_sql.rs:604-615)."""
+    q, k, v = data.attention_tables(1, T, 16, seed=T)
+    x = np.arange(T)
+    hi = x + 1
+    lo = np.maximum(0, x - window + 1) if window else np.zeros(T, np.int64)
+    out, visited, over = R.fused_core(_p(0.25), T, 16, T, 16, q[0], np.ascontiguousarray(k[0].T), v[0], lo, hi)
+    assert not over
+    want = _np_attention(q[0], k[0], v[0], 0.25, lo, hi)
+    assert np.max(np.abs(out.reshape(T, 16) - want)) <= 1e-9 * np.abs(v).max()
+
+
+def _attention_pipeline(conditions, batched):
+    """q(t, d, x), k(t, d, x), v(t, x) with an optional batch key b first
+    (pipeline_vs_sql.rs:622-690): dims 0 = i, 1 = e, 2 = j, 3 = b."""
+    b0 = 1 if batched else 0
+
+    def operand(own):
+        keys = [ir.Key(d, c + b0) for c, d in enumerate(own)]
+        if batched:
+            keys.insert(0, ir.Key(3, 0))
+        return ir.Operand(tuple(keys), len(own) + b0)
+    outputs = [ir.Output(0), ir.Output(2)]
+    kept = [0]
+    if batched:
+        outputs.insert(0, ir.Output(3))
+        kept.insert(0, 3)
+    producer = ir.Fold([EQ] * (3 + b0), [operand([0, 1]), operand([2, 1])], outputs,
+                       conditions=[ir.Condition(0, c, 2, o) for c, o in conditions])
+    return ir.Pipeline(producer, ir.Exp(), [], operand([2]), kept)
+
+
+def _attention_arrow(l, batches, q_fill, k_fill, v_fill):
+    nb = batches or 1
+
+    def table(dims, fill):
+        grids = np.meshgrid(*[np.arange(n, dtype=np.int64) for n in dims], indexing="ij")
+        cols = {f"c{i}": g.reshape(-1) for i, g in enumerate(grids)}
+        cols["v"] = fill(grids[0].size)
+        return pa.table(cols)
+    pre = [nb] if batches else []
+    return (table(pre + [l, 8], q_fill), table(pre + [l, 8], k_fill), table(pre + [l], v_fill))
+
+
+@pytest.mark.parametrize("l,batches,conditions,want", [
+    (1024, None, [(ir.Comparison.GREATER_OR_EQUAL, 0)], (136, 256)),
+    (1024, None, [(ir.Comparison.GREATER_OR_EQUAL, 0), (ir.Comparison.LESS, 64)], (31, 256)),
+    (256, 8, [(ir.Comparison.GREATER_OR_EQUAL, 0)], (80, 128)),
+])
+def test_tile_counts_known_answers(l, batches, conditions, want):
+    """pipeline_vs_sql.rs:734-760, and every average of v = 1 is 1."""
+    rng = np.random.default_rng(7)
+    q, k, v = _attention_arrow(l, batches, lambda n: rng.random(n), lambda n: np.full(n, 0.5),
+                               lambda n: np.ones(n))
+    p = _attention_pipeline(conditions, batches is not None)
+    sch = R.pipeline_schema(p, [q.schema, k.schema, v.schema])
+    piped = R.pipeline(p, q.to_batches(max_chunksize=1000), k.to_batches(), v.to_batches(), sch)
+    assert (piped.tiles_visited, piped.tiles) == want
+    t = pa.Table.from_batches(piped.batches)
+    assert t.num_rows == l * (batches or 1)
+    assert np.allclose(t.column(t.num_columns - 1).to_numpy(), 1.0, atol=1e-12)
+
+
+def test_bounds_known_cases():
+    """fused.rs:1116-1177."""
+    hold = lambda *a: bool(oracle.lib().ef_oracle_bounds_hold(*a))
+    tiny = np.finfo(float).smallest_subnormal
+
+    def hb(heaviest, lightest, largest, smallest, z, results, special=None, has_z=True):
+        res = np.array(results, np.float64)
+        sp = np.array(special if special is not None else [0] * len(results), np.uint8)
+        return hold(heaviest, lightest, largest, smallest, 3, int(has_z), z if z is not None else 0.0,
+                    oracle.ptr(res), oracle.ptr(sp), len(res))
+    # an_exact_zero_of_normal_terms_holds
+    assert hb(1.0, 0.25, 2.0, 0.5, 1.5, [0.0, 1.0])
+    assert hb(1.0, 0.25, 2.0, 0.5, None, [0.0], has_z=False)
+    assert hb(1.0, 0.25, 0.0, math.inf, 1.5, [0.0])
+    # an_exact_zero_of_terms_that_may_underflow_declines
+    assert not hb(1.0, 1.0, 1.0, 1e-300, 1e10, [0.0])
+    assert not hb(1.0, 1e-300, 1.0, 1e-10, None, [0.0], has_z=False)
+    # ordinary / negligible underflow
+    assert hb(1.0, tiny, 1e5, 1e5, 1.5, [0.3, -2.0])
+    assert hb(1.0, tiny, 1e5, 1e5, None, [0.3, -2.0], has_z=False)
+    assert hb(1.1e188, tiny, 1.0, 1.0, 1.1e188, [0.4])
+    # products / quotients that round to zero, overflow
+    assert not hb(1e-323, tiny, 0.125, 0.125, 2e-323, [0.0])
+    assert not hb(2.0, tiny, 1e300, 1e300, 2.0, [2.5e-24])
+    assert not hb(1.0, tiny, 1e100, 1e100, 1e-309, [1e100])
+    assert hb(1.0, tiny, 1e100, 1e100, None, [1e100], has_z=False)
+    assert not hb(1.0, tiny, 1e308, 1e308, None, [1e308], has_z=False)
+    # a_special_result_is_sqls
+    assert hb(1.0, tiny, 1.0, 1.0, 1.0, [0.0, math.inf], [1, 1])
+    assert not hb(1.0, tiny, 1.0, 1.0, 1.0, [0.0], [0])
+
+
+def test_expression_known_answers():
+    """crates/einfold-ir/src/expression.rs:241-252."""
+    import ctypes
+    E = ir.Expression
+    d = lambda i: E.col(0, i) - E.col(1, i)
+    dist = (d(1) ** 2.0) + (d(2) ** 2.0)
+    assert [(o, c) for o, c in dist.columns()] == [(0, 1), (0, 2), (1, 1), (1, 2)]
+    # Row value of column (o, c) is 10 o + c: (1 - 11)^2 + (2 - 12)^2 = 200.
+    from oracle.reference import _oracle_expr
+    arr, n = _oracle_expr(dist, lambda o, c: (o, c))
+    fn = oracle.lib().ef_oracle_eval
+    # Evaluate with q-side values [0, 1, 2] and k-side [10, 11, 12] through
+    # the fused oracle's evaluator (one pair).
+    p = ir.Pipeline(producer=ir.Fold([], [], [], row=dist), weight=ir.Of(E.col(0, 0)),
+                    dimensions=[], values=ir.Operand((), 0), outputs=[],
+                    normalizer=ir.Normalizer.NONE)
+    qs = [np.array([[float(c)]]) for c in range(3)]
+    ks = [np.array([[10.0 + c]]) for c in range(3)]
+    slots = {(0, c): (0, c) for c in range(3)}
+    slots.update({(1, c): (1, c) for c in range(3)})
+    out, _, over = R.fused_core(p, 1, 1, 1, 1, None, None, np.array([[1.0]]), [0], [1], (qs, ks, slots))
+    assert out[0] == 200.0 and not over
+    vals = np.array([0.0], np.float64)
+    res = np.zeros(1)
+    ln0, _ = _oracle_expr(E.lit(0.0).unary(ir.Op.LN), lambda o, c: (0, 0))
+    fn(ctypes.cast(ln0, ctypes.c_void_p), 2, oracle.ptr(vals), 1, oracle.ptr(res))
+    assert res[0] == -math.inf
+    over0, _ = _oracle_expr(E.lit(1.0) / 0.0, lambda o, c: (0, 0))
+    fn(ctypes.cast(over0, ctypes.c_void_p), 3, oracle.ptr(vals), 1, oracle.ptr(res))
+    assert res[0] == math.inf
