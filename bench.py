@@ -172,25 +172,28 @@ def main():
 
 def cpu_baseline():
     """The oracle's restatement of einfold's fused CPU kernel (fused.rs
-    tile loop, FP64, one thread) on one head of the workload, forward only
+    tile loop, FP64, one thread) on a bounded sample of the workload: the
+    last 1024 queries of one head (each sees up to 4096 keys), forward only
     (the reference has no backward)."""
-    import oracle
+    import oracle  # noqa: F401  (builds nothing: the library travels prebuilt)
     from oracle import reference as R
     from einfold_amd import data, ir
     q, k, v = data.attention_tables(1, T, D, seed=3)
-    lo = np.zeros(T, np.int64)
-    hi = np.arange(1, T + 1, dtype=np.int64)
+    x0 = T - 1024
+    lo = np.zeros(T - x0, np.int64)
+    hi = np.arange(x0 + 1, T + 1, dtype=np.int64)
     p = ir.Pipeline(producer=ir.Fold([], [], []), weight=ir.Exp(ir.Scale(ir.ScaleKind.TIMES, SCALE)),
                     dimensions=[], values=ir.Operand((), 0), outputs=[])
     kt = np.ascontiguousarray(k[0].T)
     t0 = time.perf_counter()
-    R.fused_core(p, T, D, T, D, q[0], kt, v[0], lo, hi)
+    R.fused_core(p, T - x0, D, T, D, np.ascontiguousarray(q[0, x0:]), kt, v[0], lo, hi)
     dt = time.perf_counter() - t0
-    flop = 4 * D * pairs_per_head(T)
+    flop = 4 * D * int(hi.sum())
     return {"value": round(flop / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
             "seconds": round(dt, 3),
-            "sample": "1 of 8 heads, forward only, T=4096, d=64, causal, FP64, the oracle's C "
-                      "restatement of fused.rs's tile loop on 1 thread"}
+            "sample": "queries 3072..4095 of 1 of 8 heads (3.67M of 67M allowed pairs), forward only, "
+                      "T=4096, d=64, causal, FP64, the oracle's C restatement of fused.rs's tile loop "
+                      "on 1 thread"}
 
 
 if __name__ == "__main__":

@@ -22,8 +22,8 @@ namespace {
 
 constexpr int BM = 64;   // rows per workgroup (4 waves x 16)
 constexpr int BN = 64;   // columns per tile
-constexpr float LOG2E = 1.4426950408889634f;
-constexpr float LN2 = 0.6931471805599453f;
+constexpr double LOG2E = 1.4426950408889634;   // cast to the precision's scalar
+constexpr double LN2 = 0.6931471805599453;
 
 struct Span { int64_t lo, hi; };
 
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void attn_fwd_kernel(FwdArgs a) {
                 }
             }
             __syncthreads();
-            // S^T block kb: keys j0 + kb*16 + g*4 + i, query x.
+            // S^T block kb: keys j0 + kb*16 + crow(g, i), query x.
             AccV s[4];
 #pragma unroll
             for (int kb = 0; kb < 4; kb++) {
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void attn_fwd_kernel(FwdArgs a) {
                 for (int i = 0; i < 4; i++) {
                     S t = s[kb][i] * scale2;
                     if (!full_wg) {
-                        const int64_t j = j0 + kb * 16 + g * 4 + i;
+                        const int64_t j = j0 + kb * 16 + crow<P>(g, i);
                         if (j < sp.lo || j >= sp.hi) t = neg_inf<S>();
                     }
                     s[kb][i] = t;
@@ -222,22 +222,21 @@ __global__ __launch_bounds__(256) void attn_fwd_kernel(FwdArgs a) {
 #pragma unroll
                 for (int db = 0; db < DB; db++) {
                     Frag<P> vf;
-                    const int row = (db * 16 + c) * VP + kc * 32;
-                    tile_load4x2<P>(vf, vt, VPLANE, row + g * 4, row + 16 + g * 4);
+                    tile_load_cperm<P>(vf, vt, VPLANE, (db * 16 + c) * VP + kc * 32, g);
                     mma<P>(o[db], vf, pf);
                 }
             }
             __syncthreads();
         }
     }
-    // out[x][d] for d = db*16 + g*4 + i: the weighted sums over the
+    // out[x][d] for d = db*16 + crow(g, i): the weighted sums over the
     // normalizer (a query with no keys has l = 0 and gets 0/0 = NaN).
     if (x < a.tq) {
         In *O = static_cast<In *>(a.out) + b * a.tq * D + x * D;
 #pragma unroll
         for (int db = 0; db < DB; db++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) O[db * 16 + g * 4 + i] = (In)(o[db][i] / l);
+            for (int i = 0; i < 4; i++) O[db * 16 + crow<P>(g, i)] = (In)(o[db][i] / l);
         if (g == 0) {
             a.row_max[b * a.tq + x] = (float)(m * (S)LN2);
             a.row_sum[b * a.tq + x] = (float)l;
@@ -409,7 +408,7 @@ __global__ __launch_bounds__(256) void attn_bwd_dkdv_kernel(BwdArgs a, int64_t q
         }
         __syncthreads();
         // S = Q K^T and dP = dO V^T: blocks qb, lane holds queries
-        // qb*16 + g*4 + i for key kj.
+        // qb*16 + crow(g, i) for key kj.
         AccV s[4], dp[4];
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
@@ -430,7 +429,7 @@ __global__ __launch_bounds__(256) void attn_bwd_dkdv_kernel(BwdArgs a, int64_t q
         for (int qb = 0; qb < 4; qb++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int r = qb * 16 + g * 4 + i;
+                const int r = qb * 16 + crow<P>(g, i);
                 const bool ok = kj >= slo[r] && kj < shi[r];
                 const S pv = ok ? exp2(s[qb][i] * scale2 - (S)sm[r] * (S)LOG2E) / (S)sl[r] : S(0);
                 p[qb][i] = pv;
@@ -452,20 +451,20 @@ __global__ __launch_bounds__(256) void attn_bwd_dkdv_kernel(BwdArgs a, int64_t q
             for (int db = 0; db < DB; db++) {
                 Frag<P> bf;
                 const int row = (db * 16 + c) * TP + kc * 32;
-                tile_load4x2<P>(bf, dotr, TPLANE, row + g * 4, row + 16 + g * 4);
+                tile_load_cperm<P>(bf, dotr, TPLANE, row, g);
                 mma<P>(dv[db], pf, bf);
-                tile_load4x2<P>(bf, qtr, TPLANE, row + g * 4, row + 16 + g * 4);
+                tile_load_cperm<P>(bf, qtr, TPLANE, row, g);
                 mma<P>(dk[db], dsf, bf);
             }
         }
         __syncthreads();
     }
-    // Lane holds dK/dV[key k0 + w*16 + g*4 + i][d = db*16 + c].
+    // Lane holds dK/dV[key k0 + w*16 + crow(g, i)][d = db*16 + c].
     In *DK = static_cast<In *>(a.dk) + b * a.tk * D;
     In *DV = static_cast<In *>(a.dv) + b * a.tk * D;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const int64_t key = k0 + w * 16 + g * 4 + i;
+        const int64_t key = k0 + w * 16 + crow<P>(g, i);
         if (key < a.tk) {
 #pragma unroll
             for (int db = 0; db < DB; db++) {
@@ -555,7 +554,7 @@ __global__ __launch_bounds__(256) void attn_bwd_dq_kernel(BwdArgs a) {
                 }
             }
             __syncthreads();
-            // S^T = K Q^T and dP^T = V dO^T: lane holds keys kb*16 + g*4 + i
+            // S^T = K Q^T and dP^T = V dO^T: lane holds keys kb*16 + crow(g, i)
             // for query x.
             AccV s[4], dp[4];
 #pragma unroll
@@ -576,7 +575,7 @@ __global__ __launch_bounds__(256) void attn_bwd_dq_kernel(BwdArgs a) {
             for (int kb = 0; kb < 4; kb++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const int64_t j = j0 + kb * 16 + g * 4 + i;
+                    const int64_t j = j0 + kb * 16 + crow<P>(g, i);
                     const bool ok = j >= sp.lo && j < sp.hi;
                     const S pv = ok ? exp2(s[kb][i] * scale2 - mrow) / lrow : S(0);
                     ds[kb][i] = pv * (dp[kb][i] - drow);
@@ -593,21 +592,20 @@ __global__ __launch_bounds__(256) void attn_bwd_dq_kernel(BwdArgs a) {
 #pragma unroll
                 for (int db = 0; db < DB; db++) {
                     Frag<P> kt;
-                    const int row = (db * 16 + c) * TP + kc * 32;
-                    tile_load4x2<P>(kt, ktr, TPLANE, row + g * 4, row + 16 + g * 4);
+                    tile_load_cperm<P>(kt, ktr, TPLANE, (db * 16 + c) * TP + kc * 32, g);
                     mma<P>(dq[db], kt, dsf);
                 }
             }
             __syncthreads();
         }
     }
-    // Lane holds dQ^T[d = db*16 + g*4 + i][query x].
+    // Lane holds dQ^T[d = db*16 + crow(g, i)][query x].
     if (x < a.tq) {
         In *DQ = static_cast<In *>(a.dq) + b * a.tq * D + x * D;
 #pragma unroll
         for (int db = 0; db < DB; db++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) DQ[db * 16 + g * 4 + i] = (In)(dq[db][i] * (S)a.scale);
+            for (int i = 0; i < 4; i++) DQ[db * 16 + crow<P>(g, i)] = (In)(dq[db][i] * (S)a.scale);
     }
 }
 

@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void product_kernel(const typename Input<P>::T
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t row = m0 + wm + i * 16 + g * 4 + r, col = n0 + wn + j * 16 + c;
+                const int64_t row = m0 + wm + i * 16 + crow<P>(g, r), col = n0 + wn + j * 16 + c;
                 if (row < m && col < n) Cw[row * n + col] = (In)acc[i][j][r];
             }
 }

@@ -7,10 +7,11 @@
 // of B: logical k = g*8 + j, j = 0..7.  The physical k a logical k stands
 // for is the caller's choice, as long as A and B agree:
 //   * contiguous:  physical k = g*8 + j            (operands read from memory)
-//   * cperm:       physical k = g*4 + j (j < 4), 16 + g*4 + (j-4) (j >= 4)
+//   * cperm:       physical k = crow(g, j) (j < 4), 16 + crow(g, j-4) (j >= 4)
 //                  (an operand taken from two stacked C blocks in registers)
-// The C block's layout is the same for every precision: lane l holds
-// C[g*4 + i][l % 16], i = 0..3.
+// The C block's layout: lane l holds C[crow<P>(g, i)][l % 16], i = 0..3,
+// where crow is g*4 + i for the FP32 and BF16 instructions and 4*i + g for
+// v_mfma_f64_16x16x4_f64 (its rows interleave across the lane groups).
 //
 // Precisions: FP32 issues eight v_mfma_f32_16x16x4_f32 per chunk (step t
 // takes logical k = g*8 + t from group g); FP64 likewise with
@@ -37,6 +38,11 @@ template <> struct Frag<PREC_F32> { float v[8]; };
 template <> struct Frag<PREC_F64> { double v[8]; };
 template <> struct Frag<PREC_BF16> { bf16x8 hi; };
 template <> struct Frag<PREC_BF16X3> { bf16x8 hi, lo; };
+
+// Row of a 16x16 C block held in register i of lane group g.
+template <int P> __device__ __forceinline__ constexpr int crow(int g, int i) {
+    return P == PREC_F64 ? 4 * i + g : 4 * g + i;
+}
 
 template <int P> struct Acc { typedef f32x4 type; typedef float scalar; };
 template <> struct Acc<PREC_F64> { typedef f64x4 type; typedef double scalar; };
@@ -129,26 +135,30 @@ __device__ __forceinline__ void tile_load8(Frag<P> &f, const typename Elem<P>::T
     }
 }
 
-// Loads logical k = 0..3 from base[i0 .. i0+3] and 4..7 from base[i1 .. i1+3]
-// (the cperm pattern; each index 4-element aligned).
+// Loads the cperm pattern of one 32-wide chunk starting at base[i]:
+// logical k = j holds physical crow(g, j) (j < 4) and 16 + crow(g, j - 4)
+// (j >= 4).  FP32/BF16: two runs of 4 (4-element aligned); FP64: strided.
 template <int P>
-__device__ __forceinline__ void tile_load4x2(Frag<P> &f, const typename Elem<P>::T *base, int plane, int i0, int i1) {
-    if constexpr (P == PREC_F32) {
-        const f32x4 a = *reinterpret_cast<const f32x4 *>(base + i0);
-        const f32x4 b = *reinterpret_cast<const f32x4 *>(base + i1);
+__device__ __forceinline__ void tile_load_cperm(Frag<P> &f, const typename Elem<P>::T *base, int plane, int i, int g) {
+    if constexpr (P == PREC_F64) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            f.v[j] = base[i + crow<P>(g, j)];
+            f.v[j + 4] = base[i + 16 + crow<P>(g, j)];
+        }
+    } else if constexpr (P == PREC_F32) {
+        const f32x4 a = *reinterpret_cast<const f32x4 *>(base + i + g * 4);
+        const f32x4 b = *reinterpret_cast<const f32x4 *>(base + i + 16 + g * 4);
 #pragma unroll
         for (int j = 0; j < 4; j++) { f.v[j] = a[j]; f.v[j + 4] = b[j]; }
-    } else if constexpr (P == PREC_F64) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) { f.v[j] = base[i0 + j]; f.v[j + 4] = base[i1 + j]; }
     } else {
-        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(base + i0);
-        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(base + i1);
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(base + i + g * 4);
+        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(base + i + 16 + g * 4);
 #pragma unroll
         for (int j = 0; j < 4; j++) { f.hi[j] = a[j]; f.hi[j + 4] = b[j]; }
         if constexpr (P == PREC_BF16X3) {
-            const bf16x4 c = *reinterpret_cast<const bf16x4 *>(base + plane + i0);
-            const bf16x4 d = *reinterpret_cast<const bf16x4 *>(base + plane + i1);
+            const bf16x4 c = *reinterpret_cast<const bf16x4 *>(base + plane + i + g * 4);
+            const bf16x4 d = *reinterpret_cast<const bf16x4 *>(base + plane + i + 16 + g * 4);
 #pragma unroll
             for (int j = 0; j < 4; j++) { f.lo[j] = c[j]; f.lo[j + 4] = d[j]; }
         }
@@ -156,8 +166,10 @@ __device__ __forceinline__ void tile_load4x2(Frag<P> &f, const typename Elem<P>:
 }
 
 template <class T> __device__ __forceinline__ T wave_max(T x) {
-    x = x > __shfl_xor(x, 16) ? x : __shfl_xor(x, 16);
-    x = x > __shfl_xor(x, 32) ? x : __shfl_xor(x, 32);
+    T y = __shfl_xor(x, 16);
+    x = x > y ? x : y;
+    y = __shfl_xor(x, 32);
+    x = x > y ? x : y;
     return x;
 }
 
