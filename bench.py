@@ -73,24 +73,26 @@ def main():
     def buffers(qx, dtype):
         return dict(out=torch.empty_like(qx), m=torch.empty((H, T), dtype=torch.float32, device=dev),
                     l=torch.empty((H, T), dtype=torch.float32, device=dev),
-                    dq=torch.empty_like(qx), dk=torch.empty_like(qx), dv=torch.empty_like(qx),
-                    scratch=torch.empty(H * T * 4, dtype=torch.uint8, device=dev))
+                    dq=torch.empty_like(qx), dk=torch.empty_like(qx), dv=torch.empty_like(qx))
 
     P = lambda t: ctypes.c_void_p(t.data_ptr())
 
     def step(desc, q, k, v, g, bf, kv_events=None):
         L = lib()
+        if "scratch" not in bf:
+            bf["scratch"] = device.attention_scratch(desc, dev)
+        sc = P(bf["scratch"])
         check(L.ef_attention_fwd(ctypes.byref(desc), P(q), P(k), P(v), P(bf["out"]), P(bf["m"]),
-                                 P(bf["l"]), None, stream))
-        check(L.ef_attention_bwd_prepare(ctypes.byref(desc), P(bf["out"]), P(g), P(bf["scratch"]), stream))
+                                 P(bf["l"]), None, sc, stream))
+        check(L.ef_attention_bwd_prepare(ctypes.byref(desc), P(q), P(k), P(v), P(bf["out"]), P(g), sc, stream))
         if kv_events:
             kv_events[0].record()
-        check(L.ef_attention_bwd_kv(ctypes.byref(desc), P(q), P(k), P(v), P(g), P(bf["m"]), P(bf["l"]),
-                                    P(bf["scratch"]), P(bf["dk"]), P(bf["dv"]), stream))
+        check(L.ef_attention_bwd_kv(ctypes.byref(desc), P(k), P(v), P(bf["m"]), P(bf["l"]), sc,
+                                    P(bf["dk"]), P(bf["dv"]), stream))
         if kv_events:
             kv_events[1].record()
-        check(L.ef_attention_bwd_q(ctypes.byref(desc), P(q), P(k), P(v), P(g), P(bf["m"]), P(bf["l"]),
-                                   P(bf["scratch"]), P(bf["dq"]), stream))
+        check(L.ef_attention_bwd_q(ctypes.byref(desc), P(q), P(g), P(bf["m"]), P(bf["l"]), sc,
+                                   P(bf["dq"]), stream))
 
     def timed(desc, q, k, v, g, bf, steps, warmup, kv_time=False):
         for _ in range(warmup):

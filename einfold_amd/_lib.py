@@ -26,7 +26,7 @@ EXPORTS = [
     "ef_dense_may_run", "ef_dense_run_product", "ef_dense_run_attention",
     "ef_dense_accumulate_f64", "ef_dense_product_f32",
     "ef_rank_scratch_bytes", "ef_rank_u64",
-    "ef_attention_fwd", "ef_attention_bwd_scratch_bytes", "ef_attention_bwd",
+    "ef_attention_scratch_bytes", "ef_attention_fwd", "ef_attention_bwd",
     "ef_attention_bwd_prepare", "ef_attention_bwd_kv", "ef_attention_bwd_q",
 ]
 
@@ -148,17 +148,18 @@ def lib():
         L.ef_rank_scratch_bytes.restype = sz
         L.ef_rank_u64.argtypes = [P, sz, P, P, ctypes.POINTER(sz), P, P]
         L.ef_rank_u64.restype = st
-        L.ef_attention_fwd.argtypes = [ctypes.POINTER(CAttentionDesc), P, P, P, P, P, P, P, P]
+        D = ctypes.POINTER(CAttentionDesc)
+        L.ef_attention_scratch_bytes.argtypes = [D]
+        L.ef_attention_scratch_bytes.restype = sz
+        L.ef_attention_fwd.argtypes = [D, P, P, P, P, P, P, P, P, P]
         L.ef_attention_fwd.restype = st
-        L.ef_attention_bwd_scratch_bytes.argtypes = [ctypes.POINTER(CAttentionDesc)]
-        L.ef_attention_bwd_scratch_bytes.restype = sz
-        L.ef_attention_bwd.argtypes = [ctypes.POINTER(CAttentionDesc), P, P, P, P, P, P, P, P, P, P, P, P]
+        L.ef_attention_bwd.argtypes = [D, P, P, P, P, P, P, P, P, P, P, P, P]
         L.ef_attention_bwd.restype = st
-        L.ef_attention_bwd_prepare.argtypes = [ctypes.POINTER(CAttentionDesc), P, P, P, P]
+        L.ef_attention_bwd_prepare.argtypes = [D, P, P, P, P, P, P, P]
         L.ef_attention_bwd_prepare.restype = st
-        L.ef_attention_bwd_kv.argtypes = [ctypes.POINTER(CAttentionDesc), P, P, P, P, P, P, P, P, P, P]
+        L.ef_attention_bwd_kv.argtypes = [D, P, P, P, P, P, P, P, P]
         L.ef_attention_bwd_kv.restype = st
-        L.ef_attention_bwd_q.argtypes = [ctypes.POINTER(CAttentionDesc), P, P, P, P, P, P, P, P, P]
+        L.ef_attention_bwd_q.argtypes = [D, P, P, P, P, P, P, P]
         L.ef_attention_bwd_q.restype = st
         _lib = L
     return _lib

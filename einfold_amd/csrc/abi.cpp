@@ -27,6 +27,26 @@ ef_status hip_status(hipError_t e, const char *what) {
 
 using namespace ef;
 
+namespace {
+
+// Runs `body` with the caller's scratch, or with a buffer of its own that
+// lives until the stream has finished with it.
+template <class F>
+ef_status with_scratch(const ef_attention_desc *desc, void *scratch, void *stream, F body) {
+    if (!desc) {
+        set_error("no attention description");
+        return EF_INVALID;
+    }
+    if (scratch) return body(scratch);
+    DeviceBuffer own;
+    EF_TRY(own.alloc(attention_scratch_bytes(*desc)));
+    ef_status st = body(own.ptr);
+    EF_HIP(hipStreamSynchronize(as_stream(stream)));
+    return st;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *ef_last_error(void) { return last_error.c_str(); }
@@ -60,73 +80,59 @@ ef_status ef_rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t 
     return rank_u64(words, n, rank, first, n_ranks, scratch, as_stream(stream));
 }
 
-ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
-                           void *out, float *row_max, float *row_sum, uint64_t *tiles_visited,
-                           void *stream) {
-    clear_error();
-    if (!desc) {
-        set_error("no attention description");
-        return EF_INVALID;
-    }
-    return attention_fwd(*desc, q, k, v, out, row_max, row_sum, tiles_visited, as_stream(stream));
+size_t ef_attention_scratch_bytes(const ef_attention_desc *desc) {
+    return desc ? attention_scratch_bytes(*desc) : 0;
 }
 
-size_t ef_attention_bwd_scratch_bytes(const ef_attention_desc *desc) {
-    return desc ? attention_bwd_scratch_bytes(*desc) : 0;
+ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
+                           void *out, float *row_max, float *row_sum, uint64_t *tiles_visited,
+                           void *scratch, void *stream) {
+    clear_error();
+    return with_scratch(desc, scratch, stream, [&](void *sc) {
+        return attention_fwd(*desc, q, k, v, out, row_max, row_sum, tiles_visited, sc, as_stream(stream));
+    });
 }
 
 ef_status ef_attention_bwd(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
                            const void *out, const void *dout, const float *row_max, const float *row_sum,
                            void *dq, void *dk, void *dv, void *scratch, void *stream) {
     clear_error();
-    if (!desc) {
-        set_error("no attention description");
-        return EF_INVALID;
-    }
-    DeviceBuffer own;
-    if (!scratch) {
-        EF_TRY(own.alloc(attention_bwd_scratch_bytes(*desc)));
-        scratch = own.ptr;
-    }
-    ef_status st = attention_bwd(*desc, q, k, v, out, dout, row_max, row_sum, dq, dk, dv, scratch, 7,
-                                 as_stream(stream));
-    if (own.ptr) EF_HIP(hipStreamSynchronize(as_stream(stream)));
-    return st;
+    return with_scratch(desc, scratch, stream, [&](void *sc) {
+        return attention_bwd(*desc, q, k, v, out, dout, row_max, row_sum, dq, dk, dv, sc, 7, as_stream(stream));
+    });
 }
 
-ef_status ef_attention_bwd_prepare(const ef_attention_desc *desc, const void *out, const void *dout,
-                                   void *scratch, void *stream) {
+ef_status ef_attention_bwd_prepare(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
+                                   const void *out, const void *dout, void *scratch, void *stream) {
     clear_error();
     if (!desc || !scratch) {
         set_error("a description and scratch are required");
         return EF_INVALID;
     }
-    return attention_bwd(*desc, nullptr, nullptr, nullptr, out, dout, nullptr, nullptr, nullptr, nullptr,
-                         nullptr, scratch, 1, as_stream(stream));
+    return attention_bwd(*desc, q, k, v, out, dout, nullptr, nullptr, nullptr, nullptr, nullptr, scratch, 1,
+                         as_stream(stream));
 }
 
-ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
-                              const void *dout, const float *row_max, const float *row_sum,
-                              const void *scratch, void *dk, void *dv, void *stream) {
+ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *k, const void *v, const float *row_max,
+                              const float *row_sum, void *scratch, void *dk, void *dv, void *stream) {
     clear_error();
     if (!desc || !scratch) {
         set_error("a description and the prepared scratch are required");
         return EF_INVALID;
     }
-    return attention_bwd(*desc, q, k, v, nullptr, dout, row_max, row_sum, nullptr, dk, dv,
-                         const_cast<void *>(scratch), 2, as_stream(stream));
+    return attention_bwd(*desc, nullptr, k, v, nullptr, nullptr, row_max, row_sum, nullptr, dk, dv, scratch, 2,
+                         as_stream(stream));
 }
 
-ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
-                             const void *dout, const float *row_max, const float *row_sum,
-                             const void *scratch, void *dq, void *stream) {
+ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const void *dout, const float *row_max,
+                             const float *row_sum, void *scratch, void *dq, void *stream) {
     clear_error();
     if (!desc || !scratch) {
         set_error("a description and the prepared scratch are required");
         return EF_INVALID;
     }
-    return attention_bwd(*desc, q, k, v, nullptr, dout, row_max, row_sum, dq, nullptr, nullptr,
-                         const_cast<void *>(scratch), 4, as_stream(stream));
+    return attention_bwd(*desc, q, nullptr, nullptr, nullptr, dout, row_max, row_sum, dq, nullptr, nullptr, scratch, 4,
+                         as_stream(stream));
 }
 
 // ---------------------------------------------------------------------
@@ -219,7 +225,9 @@ ef_status ef_dense_run_attention(const double *q, const double *kt, const double
     d.scale = scale_kind == EF_SCALE_TIMES ? scale : (scale_kind == EF_SCALE_OVER ? 1.0 / scale : 1.0);
     d.allowed = dsp.as<int64_t>();
     d.precision = EF_PRECISION_FP64;
-    EF_TRY(attention_fwd(d, dq.ptr, dk.ptr, dv.ptr, dout.ptr, dm.as<float>(), dl.as<float>(), nullptr, s));
+    DeviceBuffer scratch;
+    EF_TRY(scratch.alloc(attention_scratch_bytes(d)));
+    EF_TRY(attention_fwd(d, dq.ptr, dk.ptr, dv.ptr, dout.ptr, dm.as<float>(), dl.as<float>(), nullptr, scratch.ptr, s));
     EF_HIP(hipMemcpyAsync(out, dout.ptr, no * 8, hipMemcpyDeviceToHost, s));
     EF_HIP(hipStreamSynchronize(s));
     return EF_OK;

@@ -1,14 +1,21 @@
 // Fused attention pipeline on gfx950: the reference's tile algorithm
-// (crates/einfold-kernels/src/fused.rs:694-1019) with one score tile per
-// wave, an online softmax in registers, and key tiles that no query of a
+// (crates/einfold-kernels/src/fused.rs:694-1019) with the score tile in
+// registers, an online softmax, and key tiles that no query of a
 // workgroup may see never visited; plus backward kernels that recompute the
-// score tiles from the saved row max and row sum.
+// score tiles from the saved row max and row sum instead of storing them.
 //
-// Layout: q [B][Tq][D], k [B][Tk][D], v [B][Tk][DV], out [B][Tq][DV],
-// row statistics [B][Tq].  A workgroup is 4 waves; each wave owns 16 rows
-// (queries in the forward and dQ kernels, keys in the dK/dV kernel) and
-// works on 64-wide tiles of the other side, so a workgroup's tiles are the
-// reference's 64 x 64 tile pairs (fused.rs:109-110).
+// Layout: q [B][Tq][D], k [B][Tk][D], v [B][Tk][D], out [B][Tq][D], row
+// statistics [B][Tq].  A prep pass first writes the streamed operands once
+// in the form the MFMA fragments read (split into BF16 hi/lo planes for
+// BF16x3, transposed where a fragment reads along the other axis), padded
+// to whole tiles with zeros, so each tile is staged into LDS by plain
+// 16-byte copies.
+//
+// Scheduling: workgroups are persistent; each takes (batch, tile) items
+// from a counter, the heaviest first under a causal mask, so the
+// triangle's long rows don't finish last.  Each item is computed whole by
+// one workgroup, keys in order, so the bits never depend on which
+// workgroup took it.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
@@ -20,8 +27,8 @@ namespace ef {
 
 namespace {
 
-constexpr int BM = 64;   // rows per workgroup (4 waves x 16)
-constexpr int BN = 64;   // columns per tile
+constexpr int BN = 64;    // rows of a streamed tile (keys in fwd/dq, queries in dk/dv)
+constexpr int BW = 128;   // rows a workgroup owns (4 waves x 32)
 constexpr double LOG2E = 1.4426950408889634;   // cast to the precision's scalar
 constexpr double LN2 = 0.6931471805599453;
 
@@ -54,196 +61,386 @@ __device__ __forceinline__ Span query_span(const Mask &m, int64_t x) {
     return {lo, hi};
 }
 
-struct FwdArgs {
-    const void *q, *k, *v;
-    void *out;
-    float *row_max, *row_sum;
-    unsigned long long *visited;
-    Mask mask;
-    int64_t tq, tk;
-    double scale;
-    int reverse;   // visit query tiles last-first (causal load balance)
-};
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
-template <class T> __device__ __forceinline__ T neg_inf() { return -INFINITY; }
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));   // one 16-byte chunk
 
-// The forward pass for D = DV features, in precision P.
-template <int P, int D>
-__global__ __launch_bounds__(256) void attn_fwd_kernel(FwdArgs a) {
-    typedef typename Input<P>::T In;
-    typedef typename Acc<P>::type AccV;
-    typedef typename Acc<P>::scalar S;
-    typedef typename Elem<P>::T E;
-    constexpr int KC = D / 32;           // 32-wide chunks of the features
-    constexpr int DB = D / 16;           // 16-wide output blocks
-    constexpr int KP = D + 8;            // K tile row pitch (elements)
-    constexpr int VP = BN + 8;           // V^T tile row pitch
-    constexpr int KPLANE = BN * KP;
-    constexpr int VPLANE = D * VP;
-    __shared__ __attribute__((aligned(16))) E ks[Elem<P>::planes * KPLANE];
-    __shared__ __attribute__((aligned(16))) E vt[Elem<P>::planes * VPLANE];
-    __shared__ int64_t red_lo[4], red_hi[4];
-    __shared__ int flags[2];
-
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
-    const int64_t nqt = (a.tq + BM - 1) / BM;
-    const int64_t qt = a.reverse ? nqt - 1 - blockIdx.x : blockIdx.x;
-    const int64_t b = blockIdx.y;
-    const int64_t q0 = qt * BM;
-    const int64_t x = q0 + w * 16 + c;   // this lane's query
-    const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
-    const In *K = static_cast<const In *>(a.k) + b * a.tk * D;
-    const In *V = static_cast<const In *>(a.v) + b * a.tk * D;
-
-    const Span sp = query_span(a.mask, x);
-    // The workgroup's key range: the union of its queries' spans.
-    {
-        int64_t lo = sp.lo < sp.hi ? sp.lo : INT64_MAX;
-        int64_t hi = sp.lo < sp.hi ? sp.hi : 0;
-        for (int o = 32; o >= 1; o >>= 1) {
-            int64_t l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
-            lo = l2 < lo ? l2 : lo;
-            hi = h2 > hi ? h2 : hi;
-        }
-        if (lane == 0) { red_lo[w] = lo; red_hi[w] = hi; }
+// One value into an operand array of `plane` elements per plane (BF16x3
+// writes hi and lo planes).
+template <int P, class T>
+__device__ __forceinline__ void put_split(typename Elem<P>::T *base, int64_t plane, int64_t idx, T x) {
+    if constexpr (P == PREC_BF16X3) {
+        __bf16 h, l;
+        split_bf16((float)x, h, l);
+        base[idx] = h;
+        base[plane + idx] = l;
+    } else {
+        base[idx] = (typename Elem<P>::T)x;
     }
+}
+
+// ---------------------------------------------------------------------
+// Prep: x [B][T][D] into row-major planes [planes][B][Tp][D] and/or
+// transposed planes [planes][B][D][Tp], zero past T.  One block per 64
+// rows of one batch.
+// ---------------------------------------------------------------------
+template <int P, int D>
+__global__ __launch_bounds__(256) void prep_kernel(const typename Input<P>::T *__restrict__ x, int64_t T, int64_t Tp,
+                                                   int64_t B, typename Elem<P>::T *rows, typename Elem<P>::T *trans) {
+    typedef typename Input<P>::T In;
+    __shared__ In t[64][D + 1];
+    const int64_t b = blockIdx.y;
+    const int64_t t0 = (int64_t)blockIdx.x * 64;
+    const int64_t plane = B * Tp * D;
+    for (int i = threadIdx.x; i < 64 * D; i += 256) {
+        const int r = i / D, c = i % D;
+        const int64_t tt = t0 + r;
+        const In v = tt < T ? x[(b * T + tt) * D + c] : In(0);
+        t[r][c] = v;
+        if (rows) put_split<P>(rows, plane, (b * Tp + tt) * D + c, v);
+    }
+    if (!trans) return;
     __syncthreads();
-    int64_t first = red_lo[0], last = red_hi[0];
+    for (int i = threadIdx.x; i < 64 * D; i += 256) {
+        const int c = i / 64, r = i % 64;
+        put_split<P>(trans, plane, (b * D + c) * Tp + t0 + r, t[r][c]);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Work counters: one per kernel, zeroed by a small kernel before each
+// launch (plain stores, so the pair may be captured in a graph).
+// ---------------------------------------------------------------------
+struct Counter { unsigned next, done; };
+
+__global__ void zero_counters_kernel(Counter *c) {
+    if (threadIdx.x < 4) { c[threadIdx.x].next = 0; c[threadIdx.x].done = 0; }
+}
+
+__device__ __forceinline__ int take_item(Counter *ctr, int *slot) {
+    if (threadIdx.x == 0) *slot = (int)atomicAdd(&ctr->next, 1u);
+    __syncthreads();
+    const int item = *slot;
+    __syncthreads();
+    return item;
+}
+
+// Copies `rows` rows of `cols` elements (16-byte chunks) of every plane
+// from global (row stride gstride, plane stride gplane) into an LDS tile
+// (row pitch lpitch, plane stride lplane).
+template <class E>
+__device__ __forceinline__ void stage(E *lds, int lpitch, int lplane, const E *g, int64_t gstride, int64_t gplane,
+                                      int rows, int cols, int planes) {
+    constexpr int CE = 16 / sizeof(E);
+    const int per_row = cols / CE;
+    const int per_plane = rows * per_row;
+    const int total = planes * per_plane;
+    for (int k = threadIdx.x; k < total; k += 256) {
+        const int p = k / per_plane, rem = k % per_plane;
+        const int r = rem / per_row, c = (rem % per_row) * CE;
+        const u32x4 v = *reinterpret_cast<const u32x4 *>(g + p * gplane + r * gstride + c);
+        *reinterpret_cast<u32x4 *>(lds + p * lplane + r * lpitch + c) = v;
+    }
+}
+
+// The union of the spans of a workgroup's queries (two per lane).
+__device__ __forceinline__ void group_range(const Span (&sp)[2], int64_t *red_lo, int64_t *red_hi, int64_t &first,
+                                            int64_t &last) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t lo = INT64_MAX, hi = 0;
+#pragma unroll
+    for (int qb = 0; qb < 2; qb++)
+        if (sp[qb].lo < sp[qb].hi) {
+            lo = sp[qb].lo < lo ? sp[qb].lo : lo;
+            hi = sp[qb].hi > hi ? sp[qb].hi : hi;
+        }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int64_t l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if (lane == 0) { red_lo[w] = lo; red_hi[w] = hi; }
+    __syncthreads();
+    first = red_lo[0];
+    last = red_hi[0];
     for (int i = 1; i < 4; i++) {
         first = red_lo[i] < first ? red_lo[i] : first;
         last = red_hi[i] > last ? red_hi[i] : last;
     }
+    __syncthreads();
+}
 
-    // This lane's queries as B fragments of S^T = K Q^T (column = query).
-    Frag<P> qf[KC];
-#pragma unroll
-    for (int kc = 0; kc < KC; kc++) {
-        In tmp[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) tmp[j] = x < a.tq ? Q[x * D + kc * 32 + g * 8 + j] : In(0);
-        make_frag<P>(qf[kc], tmp);
-    }
+// ---------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------
+struct FwdArgs {
+    const void *q;
+    const void *kr, *vt;          // prepared K rows and V^T
+    void *out;
+    float *row_max, *row_sum;
+    unsigned long long *visited;
+    Counter *ctr;
+    Mask mask;
+    int64_t tq, tk, tkp, batches;
+    double scale;
+    int items, reverse;
+};
 
+// The K and V^T tiles of one 64-key step, as 16-byte chunks: thread t
+// moves chunks t, t + 256, ...; the first KN of each thread's are K's.
+template <int P, int D> struct FwdTiles {
+    typedef typename Elem<P>::T E;
+    static constexpr int PL = Elem<P>::planes;
+    static constexpr int CE = 16 / sizeof(E);
+    static constexpr int KP = D + 8, VP = BN + 8;
+    static constexpr int KPLANE = BN * KP, VPLANE = D * VP;
+    static constexpr int KCH = PL * BN * (D / CE);     // K chunks
+    static constexpr int VCH = PL * D * (BN / CE);     // V^T chunks
+    static constexpr int KN = KCH / 256, VN = VCH / 256, N = KN + VN;
+    static constexpr bool prefetch = N <= 8;           // hold the next tile in registers
+    static constexpr int BUF = PL * (KPLANE + VPLANE); // elements per LDS buffer
+};
+
+template <int P, int D>
+__global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
+    typedef typename Input<P>::T In;
+    typedef typename Acc<P>::type AccV;
+    typedef typename Acc<P>::scalar S;
+    typedef typename Elem<P>::T E;
+    typedef FwdTiles<P, D> F;
+    constexpr int KC = D / 32, DB = D / 16, NP = Parts<P>::n;
+    constexpr int NBUF = F::prefetch ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) E lds[NBUF * F::BUF];
+    __shared__ int64_t red_lo[4], red_hi[4];
+    __shared__ int flags[2][4];   // per buffer, per wave: bit 0 any, bit 1 full
+    __shared__ int slot;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
+    const int64_t nqt = (a.tq + BW - 1) / BW;
+    const int64_t kplane = a.batches * a.tkp * D, vplane = a.batches * D * a.tkp;
     const S scale2 = (S)a.scale * (S)LOG2E;   // scores in log2 units
-    S m = neg_inf<S>(), l = 0;
-    AccV o[DB];
-#pragma unroll
-    for (int i = 0; i < DB; i++) o[i] = AccV{0, 0, 0, 0};
-    unsigned visited = 0;
+    unsigned long long visited = 0;
 
-    if (first < last) {
-        for (int64_t j0 = first - first % BN; j0 < last; j0 += BN) {
-            // Whether any query sees this tile, and whether every query
-            // sees all of it (then nothing is masked).
-            const bool any = sp.lo < sp.hi && sp.lo < j0 + BN && sp.hi > j0;
-            const bool full = x >= a.tq || (sp.lo <= j0 && sp.hi >= j0 + BN);
-            const int any_wg = __syncthreads_or(any);
-            const int full_wg = __syncthreads_and(full);
-            if (!any_wg) continue;
-            visited++;
-            // Stage K [key][d] and V^T [d][key] (zero past the keys).
-            for (int idx = tid * 4; idx < BN * D; idx += 256 * 4) {
-                const int r = idx / D, col = idx % D;
-                const int64_t j = j0 + r;
-                In kv[4], vv[4];
+    // This thread's chunks: global offsets from the tile's base, LDS offsets
+    // from the buffer's (loop-invariant).
+    int goff[F::N];   // fits: a plane of prepared operands stays below 2^31 elements (checked on the host)
+    int loff[F::N];
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    kv[i] = j < a.tk ? K[j * D + col + i] : In(0);
-                    vv[i] = j < a.tk ? V[j * D + col + i] : In(0);
-                }
+    for (int i = 0; i < F::N; i++) {
+        if (i < F::KN) {
+            const int k = tid + 256 * i, per_row = D / F::CE, per_plane = BN * per_row;
+            const int pl = k / per_plane, r = (k % per_plane) / per_row, col = (k % per_row) * F::CE;
+            goff[i] = (int)(pl * kplane + (int64_t)r * D + col);
+            loff[i] = pl * F::KPLANE + r * F::KP + col;
+        } else {
+            const int k = tid + 256 * (i - F::KN), per_row = BN / F::CE, per_plane = D * per_row;
+            const int pl = k / per_plane, r = (k % per_plane) / per_row, col = (k % per_row) * F::CE;
+            goff[i] = (int)(pl * vplane + (int64_t)r * a.tkp + col);
+            loff[i] = F::PL * F::KPLANE + pl * F::VPLANE + r * F::VP + col;
+        }
+    }
+    for (;;) {
+        const int item = take_item(a.ctr, &slot);
+        if (item >= a.items) break;
+        const int64_t b = item % a.batches;
+        const int64_t qt = a.reverse ? nqt - 1 - item / a.batches : item / a.batches;
+        const int64_t q0 = qt * BW;
+        const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
+        const E *K = static_cast<const E *>(a.kr) + b * a.tkp * D;
+        const E *VT = static_cast<const E *>(a.vt) + b * D * a.tkp;
+
+        int64_t x[2];
+        Span sp[2];
+        Frag<P> qf[2][KC];
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    tile_store<P>(ks, KPLANE, r * KP + col + i, kv[i]);
-                    tile_store<P>(vt, VPLANE, (col + i) * VP + r, vv[i]);
-                }
+        for (int qb = 0; qb < 2; qb++) {
+            x[qb] = q0 + w * 32 + qb * 16 + c;
+            sp[qb] = query_span(a.mask, x[qb]);
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) {
+                In tmp[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) tmp[j] = x[qb] < a.tq ? Q[x[qb] * D + kc * 32 + g * 8 + j] : In(0);
+                make_frag<P>(qf[qb][kc], tmp);
             }
+        }
+        int64_t first, last;
+        group_range(sp, red_lo, red_hi, first, last);
+        const int64_t jbeg = first < last ? first - first % BN : last;
+
+        S m[2], l[2];
+        AccV o[2][DB];
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++) {
+            m[qb] = -INFINITY;
+            l[qb] = 0;
+#pragma unroll
+            for (int i = 0; i < DB; i++) o[qb][i] = AccV{0, 0, 0, 0};
+        }
+// Each wave's flags for the tile at J0: bit 0 some query sees it, bit 1
+// every query sees all of it.
+#define EF_WAVE_FLAGS(J0)                                                                        \
+    ([&]() {                                                                                     \
+        bool any_ = false, full_ = true;                                                         \
+        for (int qb_ = 0; qb_ < 2; qb_++) {                                                      \
+            any_ |= sp[qb_].lo < sp[qb_].hi && sp[qb_].lo < (J0) + BN && sp[qb_].hi > (J0);      \
+            full_ &= x[qb_] >= a.tq || (sp[qb_].lo <= (J0) && sp[qb_].hi >= (J0) + BN);         \
+        }                                                                                        \
+        return (__any(any_) ? 1 : 0) | (__all(full_) ? 2 : 0);                                   \
+    }())
+#define EF_LOAD_TILE(J0)                                                                         \
+    _Pragma("unroll") for (int i_ = 0; i_ < F::N; i_++)                                          \
+        pre[i_] = *reinterpret_cast<const u32x4 *>((i_ < F::KN ? K + (J0) * D : VT + (J0)) + goff[i_])
+#define EF_STORE_TILE(BUF)                                                                       \
+    _Pragma("unroll") for (int i_ = 0; i_ < F::N; i_++)                                          \
+        *reinterpret_cast<u32x4 *>((BUF) + loff[i_]) = pre[i_]
+        if (jbeg < last) {
+            if (F::prefetch) {
+                u32x4 pre[F::N];
+                EF_LOAD_TILE(jbeg);
+                EF_STORE_TILE(lds);
+            }
+            const int f0 = EF_WAVE_FLAGS(jbeg);
+            if (lane == 0) flags[0][w] = f0;
             __syncthreads();
-            // S^T block kb: keys j0 + kb*16 + crow(g, i), query x.
-            AccV s[4];
-#pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                s[kb] = AccV{0, 0, 0, 0};
-#pragma unroll
-                for (int kc = 0; kc < KC; kc++) {
-                    Frag<P> kf;
-                    tile_load8<P>(kf, ks, KPLANE, (kb * 16 + c) * KP + kc * 32 + g * 8);
-                    mma<P>(s[kb], kf, qf[kc]);
+        }
+        int cur = 0;
+        for (int64_t j0 = jbeg; j0 < last; j0 += BN) {
+            const bool more = j0 + BN < last;
+            u32x4 pre[F::N];
+            if (F::prefetch) {
+                if (more) {
+                    EF_LOAD_TILE(j0 + BN);
+                    const int fn = EF_WAVE_FLAGS(j0 + BN);
+                    if (lane == 0) flags[cur ^ 1][w] = fn;
                 }
-            }
-            // Online softmax for query x over this tile.
-            S mx = neg_inf<S>();
-#pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    S t = s[kb][i] * scale2;
-                    if (!full_wg) {
-                        const int64_t j = j0 + kb * 16 + crow<P>(g, i);
-                        if (j < sp.lo || j >= sp.hi) t = neg_inf<S>();
-                    }
-                    s[kb][i] = t;
-                    mx = t > mx ? t : mx;
-                }
-            }
-            mx = wave_max(mx);
-            const S mnew = mx > m ? mx : m;
-            S p[4][4];
-            S rs = 0;
-            if (mnew == neg_inf<S>()) {
-#pragma unroll
-                for (int kb = 0; kb < 4; kb++)
-#pragma unroll
-                    for (int i = 0; i < 4; i++) p[kb][i] = 0;
             } else {
+                // One buffer: stage this tile now.
+                stage<E>(lds, F::KP, F::KPLANE, K + j0 * D, D, kplane, BN, D, F::PL);
+                stage<E>(lds + F::PL * F::KPLANE, F::VP, F::VPLANE, VT + j0, a.tkp, vplane, D, BN, F::PL);
+                __syncthreads();
+            }
+            const int f[4] = {flags[cur][0], flags[cur][1], flags[cur][2], flags[cur][3]};
+            const bool full_wg = (f[0] & f[1] & f[2] & f[3] & 2) != 0;
+            // Tiles in the reference's 64 x 64 units: each half of the
+            // workgroup's 128 queries counts the tiles it sees.
+            if (tid == 0) visited += (unsigned)(((f[0] | f[1]) & 1) != 0) + (unsigned)(((f[2] | f[3]) & 1) != 0);
+            const E *ks = lds + (F::prefetch ? cur * F::BUF : 0);
+            const E *vt = ks + F::PL * F::KPLANE;
+
+            AccV s[2][4];
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) s[qb][kb] = AccV{0, 0, 0, 0};
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) {
+                Frag<P> kf[4];
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) tile_load8<P>(kf[kb], ks, F::KPLANE, (kb * 16 + c) * F::KP + kc * 32 + g * 8);
+#pragma unroll
+                for (int sp_ = 0; sp_ < NP; sp_++)
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+                        for (int qb = 0; qb < 2; qb++) mma_part<P>(s[qb][kb], kf[kb], qf[qb][kc], sp_);
+            }
+            // Online softmax per query column, the weights written over the
+            // scores.
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++) {
+                S mx = -INFINITY;
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++)
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
-                        p[kb][i] = exp2(s[kb][i] - mnew);
-                        rs += p[kb][i];
+                        S t = s[qb][kb][i] * scale2;
+                        if (!full_wg) {
+                            const int64_t j = j0 + kb * 16 + crow<P>(g, i);
+                            if (j < sp[qb].lo || j >= sp[qb].hi) t = -INFINITY;
+                        }
+                        s[qb][kb][i] = t;
+                        mx = fmax(mx, t);
                     }
-                const S alpha = exp2(m - mnew);   // 0 when m was -inf
+                mx = wave_max(mx);
+                const S mnew = fmax(mx, m[qb]);
+                S rs = 0;
+                if (mnew == -INFINITY) {
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++) s[qb][kb] = AccV{0, 0, 0, 0};
+                } else {
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            s[qb][kb][i] = fast_exp2(s[qb][kb][i] - mnew);
+                            rs += s[qb][kb][i];
+                        }
+                }
                 rs = group_sum(rs);
-                l = l * alpha + rs;
+                if (mnew != -INFINITY) {
+                    const S alpha = fast_exp2(m[qb] - mnew);   // 0 when m was -inf
+                    l[qb] = l[qb] * alpha + rs;
 #pragma unroll
-                for (int db = 0; db < DB; db++)
+                    for (int db = 0; db < DB; db++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) o[db][i] *= alpha;
-                m = mnew;
-            }
-            // O^T += V^T P^T: P^T's 32-key chunks from two stacked S blocks.
-#pragma unroll
-            for (int kc = 0; kc < 2; kc++) {
-                Frag<P> pf;
-                S tmp[8];
-#pragma unroll
-                for (int i = 0; i < 4; i++) { tmp[i] = p[2 * kc][i]; tmp[i + 4] = p[2 * kc + 1][i]; }
-                make_frag<P>(pf, tmp);
-#pragma unroll
-                for (int db = 0; db < DB; db++) {
-                    Frag<P> vf;
-                    tile_load_cperm<P>(vf, vt, VPLANE, (db * 16 + c) * VP + kc * 32, g);
-                    mma<P>(o[db], vf, pf);
+                        for (int i = 0; i < 4; i++) o[qb][db][i] *= alpha;
+                    m[qb] = mnew;
                 }
             }
+            // O^T += V^T P^T, P^T's 32-key chunks from two stacked S blocks.
+#pragma unroll
+            for (int kc = 0; kc < 2; kc++) {
+                Frag<P> pf[2];
+#pragma unroll
+                for (int qb = 0; qb < 2; qb++) {
+                    S tmp[8];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) { tmp[i] = s[qb][2 * kc][i]; tmp[i + 4] = s[qb][2 * kc + 1][i]; }
+                    make_frag<P>(pf[qb], tmp);
+                }
+                Frag<P> vf[DB];
+#pragma unroll
+                for (int db = 0; db < DB; db++) tile_load_cperm<P>(vf[db], vt, F::VPLANE, (db * 16 + c) * F::VP + kc * 32, g);
+#pragma unroll
+                for (int sp_ = 0; sp_ < NP; sp_++)
+#pragma unroll
+                    for (int db = 0; db < DB; db++)
+#pragma unroll
+                        for (int qb = 0; qb < 2; qb++) mma_part<P>(o[qb][db], vf[db], pf[qb], sp_);
+            }
+            if (F::prefetch && more) {
+                EF_STORE_TILE(lds + (cur ^ 1) * F::BUF);
+            }
             __syncthreads();
+            if (F::prefetch) cur ^= 1;
+            else if (more) {
+                const int fn = EF_WAVE_FLAGS(j0 + BN);
+                if (lane == 0) flags[0][w] = fn;
+                __syncthreads();
+            }
+        }
+        // out[x][d] for d = db*16 + crow(g, i): the weighted sums over the
+        // normalizer (a query with no keys has l = 0 and gets 0/0 = NaN).
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++) {
+            if (x[qb] >= a.tq) continue;
+            In *O = static_cast<In *>(a.out) + (b * a.tq + x[qb]) * D;
+#pragma unroll
+            for (int db = 0; db < DB; db++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) O[db * 16 + crow<P>(g, i)] = (In)(o[qb][db][i] / l[qb]);
+            if (g == 0) {
+                a.row_max[b * a.tq + x[qb]] = (float)(m[qb] * (S)LN2);
+                a.row_sum[b * a.tq + x[qb]] = (float)l[qb];
+            }
         }
     }
-    // out[x][d] for d = db*16 + crow(g, i): the weighted sums over the
-    // normalizer (a query with no keys has l = 0 and gets 0/0 = NaN).
-    if (x < a.tq) {
-        In *O = static_cast<In *>(a.out) + b * a.tq * D + x * D;
-#pragma unroll
-        for (int db = 0; db < DB; db++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) O[db * 16 + crow<P>(g, i)] = (In)(o[db][i] / l);
-        if (g == 0) {
-            a.row_max[b * a.tq + x] = (float)(m * (S)LN2);
-            a.row_sum[b * a.tq + x] = (float)l;
-        }
-    }
-    if (tid == 0 && a.visited) atomicAdd(a.visited, (unsigned long long)visited);
+    if (tid == 0 && a.visited && visited) atomicAdd(a.visited, visited);
 }
+#undef EF_WAVE_FLAGS
+#undef EF_LOAD_TILE
+#undef EF_STORE_TILE
 
 // Any shape, one thread per (query, block of 8 value columns), keys in
 // order: the reference's algorithm with a tile of one row.  Float64 or
@@ -296,316 +493,348 @@ __global__ __launch_bounds__(256) void attn_fwd_generic(const T *Q, const T *K, 
 // Backward
 // ---------------------------------------------------------------------
 struct BwdArgs {
-    const void *q, *k, *v, *out, *dout;
-    const float *row_max, *row_sum;
-    float *delta;
+    const void *q, *k, *v;             // originals (register fragments)
+    const void *dout;
+    const void *qr, *qt, *dor, *dot;   // prepared query-side tiles
+    const void *kr, *kt, *vr;          // prepared key-side tiles
+    const float *row_max, *row_sum, *delta;
     void *dq, *dk, *dv;
+    Counter *ctr;
     Mask mask;
-    int64_t tq, tk;
+    int64_t tq, tk, tqp, tkp, batches;
     double scale;
+    int items, reverse, causal_only, pad;
 };
 
-// delta[b][x] = sum_d dout[b][x][d] * out[b][x][d].
+// delta[b][x] = sum_d dout[b][x][d] * out[b][x][d], a fixed-order tree.
 template <class In, int D>
-__global__ __launch_bounds__(256) void attn_delta_kernel(BwdArgs a, int64_t rows) {
+__global__ __launch_bounds__(256) void attn_delta_kernel(const In *out, const In *dout, float *delta, int64_t rows) {
     const int64_t r = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int c = threadIdx.x & 15;
     float s = 0;
-    if (r < rows) {
-        const In *o = static_cast<const In *>(a.out) + r * D;
-        const In *d = static_cast<const In *>(a.dout) + r * D;
-        for (int e = c; e < D; e += 16) s += (float)o[e] * (float)d[e];
-    }
-    // Fixed-order tree over the 16 lanes of the row.
+    if (r < rows)
+        for (int e = c; e < D; e += 16) s += (float)out[r * D + e] * (float)dout[r * D + e];
     for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
-    if (r < rows && c == 0) a.delta[r] = s;
+    if (r < rows && c == 0) delta[r] = s;
 }
 
-// dK and dV for 64 keys per workgroup (16 per wave), visiting the query
+// dK and dV for 128 keys per workgroup (32 per wave), visiting the query
 // tiles that may see them.
 template <int P, int D>
-__global__ __launch_bounds__(256) void attn_bwd_dkdv_kernel(BwdArgs a, int64_t qt_begin, int64_t qt_end) {
+__global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     typedef typename Input<P>::T In;
     typedef typename Acc<P>::type AccV;
     typedef typename Acc<P>::scalar S;
     typedef typename Elem<P>::T E;
-    constexpr int KC = D / 32, DB = D / 16;
-    constexpr int RP = D + 8;    // row-major tile pitch [q][d]
-    constexpr int TP = BM + 8;   // transposed tile pitch [d][q]
-    constexpr int RPLANE = BM * RP, TPLANE = D * TP;
-    __shared__ __attribute__((aligned(16))) E qs[Elem<P>::planes * RPLANE];
-    __shared__ __attribute__((aligned(16))) E qtr[Elem<P>::planes * TPLANE];
-    __shared__ __attribute__((aligned(16))) E dos[Elem<P>::planes * RPLANE];
-    __shared__ __attribute__((aligned(16))) E dotr[Elem<P>::planes * TPLANE];
-    __shared__ float sm[BM], sl[BM], sd[BM];
-    __shared__ int64_t slo[BM], shi[BM];
-
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
-    const int64_t b = blockIdx.y;
-    const int64_t k0 = (int64_t)blockIdx.x * BM;
-    const int64_t kj = k0 + w * 16 + c;   // this lane's key (as a column)
-    const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
-    const In *K = static_cast<const In *>(a.k) + b * a.tk * D;
-    const In *V = static_cast<const In *>(a.v) + b * a.tk * D;
-    const In *DO = static_cast<const In *>(a.dout) + b * a.tq * D;
-    const float *RM = a.row_max + b * a.tq, *RS = a.row_sum + b * a.tq, *DL = a.delta + b * a.tq;
-
-    Frag<P> kf[KC], vf[KC];
-#pragma unroll
-    for (int kc = 0; kc < KC; kc++) {
-        In tk8[8], tv8[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            tk8[j] = kj < a.tk ? K[kj * D + kc * 32 + g * 8 + j] : In(0);
-            tv8[j] = kj < a.tk ? V[kj * D + kc * 32 + g * 8 + j] : In(0);
-        }
-        make_frag<P>(kf[kc], tk8);
-        make_frag<P>(vf[kc], tv8);
-    }
-    AccV dk[DB], dv[DB];
-#pragma unroll
-    for (int i = 0; i < DB; i++) { dk[i] = AccV{0, 0, 0, 0}; dv[i] = AccV{0, 0, 0, 0}; }
-    const S scale2 = (S)a.scale * (S)LOG2E;
-
-    // Under a causal mask alone, query x sees key j when j <= x + offset,
-    // so no query below k0 - offset sees this workgroup's keys.
-    if (!a.mask.allowed && a.mask.causal && a.mask.window <= 0) {
-        const int64_t xmin = k0 - a.mask.offset;
-        const int64_t first_tile = xmin > 0 ? xmin / BM : 0;
-        qt_begin = first_tile > qt_begin ? first_tile : qt_begin;
-    }
-    for (int64_t qt = qt_begin; qt < qt_end; qt++) {
-        const int64_t q0 = qt * BM;
-        if (tid < BM) {
-            const int64_t x = q0 + tid;
-            const Span sp = query_span(a.mask, x);
-            slo[tid] = sp.lo;
-            shi[tid] = sp.hi;
-            sm[tid] = x < a.tq ? RM[x] : 0.f;
-            sl[tid] = x < a.tq ? RS[x] : 1.f;
-            sd[tid] = x < a.tq ? DL[x] : 0.f;
-        }
-        __syncthreads();
-        // Whether any query of the tile sees any key of this workgroup.
-        bool any = false;
-        if (tid < BM) {
-            const int64_t kend = k0 + BM < a.tk ? k0 + BM : a.tk;
-            any = slo[tid] < shi[tid] && slo[tid] < kend && shi[tid] > k0;
-        }
-        if (!__syncthreads_or(any)) continue;
-        for (int idx = tid * 4; idx < BM * D; idx += 256 * 4) {
-            const int r = idx / D, col = idx % D;
-            const int64_t x = q0 + r;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const In qv = x < a.tq ? Q[x * D + col + i] : In(0);
-                const In dv_ = x < a.tq ? DO[x * D + col + i] : In(0);
-                tile_store<P>(qs, RPLANE, r * RP + col + i, qv);
-                tile_store<P>(qtr, TPLANE, (col + i) * TP + r, qv);
-                tile_store<P>(dos, RPLANE, r * RP + col + i, dv_);
-                tile_store<P>(dotr, TPLANE, (col + i) * TP + r, dv_);
-            }
-        }
-        __syncthreads();
-        // S = Q K^T and dP = dO V^T: blocks qb, lane holds queries
-        // qb*16 + crow(g, i) for key kj.
-        AccV s[4], dp[4];
-#pragma unroll
-        for (int qb = 0; qb < 4; qb++) {
-            s[qb] = AccV{0, 0, 0, 0};
-            dp[qb] = AccV{0, 0, 0, 0};
-#pragma unroll
-            for (int kc = 0; kc < KC; kc++) {
-                Frag<P> af;
-                tile_load8<P>(af, qs, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
-                mma<P>(s[qb], af, kf[kc]);
-                tile_load8<P>(af, dos, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
-                mma<P>(dp[qb], af, vf[kc]);
-            }
-        }
-        // P = exp(scale s - m) / l and dS = P (dP - delta), masked to 0.
-        S p[4][4], ds[4][4];
-#pragma unroll
-        for (int qb = 0; qb < 4; qb++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int r = qb * 16 + crow<P>(g, i);
-                const bool ok = kj >= slo[r] && kj < shi[r];
-                const S pv = ok ? exp2(s[qb][i] * scale2 - (S)sm[r] * (S)LOG2E) / (S)sl[r] : S(0);
-                p[qb][i] = pv;
-                ds[qb][i] = pv * (dp[qb][i] - (S)sd[r]);
-            }
-        // dV += P^T dO and dK += dS^T Q over the tile's 64 queries.
-#pragma unroll
-        for (int kc = 0; kc < 2; kc++) {
-            Frag<P> pf, dsf;
-            S t1[8], t2[8];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                t1[i] = p[2 * kc][i]; t1[i + 4] = p[2 * kc + 1][i];
-                t2[i] = ds[2 * kc][i]; t2[i + 4] = ds[2 * kc + 1][i];
-            }
-            make_frag<P>(pf, t1);
-            make_frag<P>(dsf, t2);
-#pragma unroll
-            for (int db = 0; db < DB; db++) {
-                Frag<P> bf;
-                const int row = (db * 16 + c) * TP + kc * 32;
-                tile_load_cperm<P>(bf, dotr, TPLANE, row, g);
-                mma<P>(dv[db], pf, bf);
-                tile_load_cperm<P>(bf, qtr, TPLANE, row, g);
-                mma<P>(dk[db], dsf, bf);
-            }
-        }
-        __syncthreads();
-    }
-    // Lane holds dK/dV[key k0 + w*16 + crow(g, i)][d = db*16 + c].
-    In *DK = static_cast<In *>(a.dk) + b * a.tk * D;
-    In *DV = static_cast<In *>(a.dv) + b * a.tk * D;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int64_t key = k0 + w * 16 + crow<P>(g, i);
-        if (key < a.tk) {
-#pragma unroll
-            for (int db = 0; db < DB; db++) {
-                DK[key * D + db * 16 + c] = (In)(dk[db][i] * (S)a.scale);
-                DV[key * D + db * 16 + c] = (In)dv[db][i];
-            }
-        }
-    }
-}
-
-// dQ for 64 queries per workgroup (16 per wave), visiting the key tiles
-// each may see.
-template <int P, int D>
-__global__ __launch_bounds__(256) void attn_bwd_dq_kernel(BwdArgs a) {
-    typedef typename Input<P>::T In;
-    typedef typename Acc<P>::type AccV;
-    typedef typename Acc<P>::scalar S;
-    typedef typename Elem<P>::T E;
+    constexpr int PL = Elem<P>::planes;
     constexpr int KC = D / 32, DB = D / 16;
     constexpr int RP = D + 8, TP = BN + 8;
     constexpr int RPLANE = BN * RP, TPLANE = D * TP;
-    __shared__ __attribute__((aligned(16))) E ks[Elem<P>::planes * RPLANE];
-    __shared__ __attribute__((aligned(16))) E ktr[Elem<P>::planes * TPLANE];
-    __shared__ __attribute__((aligned(16))) E vs[Elem<P>::planes * RPLANE];
-    __shared__ int64_t red_lo[4], red_hi[4];
+    __shared__ __attribute__((aligned(16))) E qs[PL * RPLANE];
+    __shared__ __attribute__((aligned(16))) E qtr[PL * TPLANE];
+    __shared__ __attribute__((aligned(16))) E dos[PL * RPLANE];
+    __shared__ __attribute__((aligned(16))) E dotr[PL * TPLANE];
+    __shared__ float sm[BN], sl[BN], sd[BN];
+    __shared__ int64_t slo[BN], shi[BN];
+    __shared__ int slot;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
-    const int64_t b = blockIdx.y;
-    const int64_t q0 = (int64_t)blockIdx.x * BM;
-    const int64_t x = q0 + w * 16 + c;
-    const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
-    const In *K = static_cast<const In *>(a.k) + b * a.tk * D;
-    const In *V = static_cast<const In *>(a.v) + b * a.tk * D;
-    const In *DO = static_cast<const In *>(a.dout) + b * a.tq * D;
-
-    const Span sp = query_span(a.mask, x);
-    {
-        int64_t lo = sp.lo < sp.hi ? sp.lo : INT64_MAX;
-        int64_t hi = sp.lo < sp.hi ? sp.hi : 0;
-        for (int o = 32; o >= 1; o >>= 1) {
-            int64_t l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
-            lo = l2 < lo ? l2 : lo;
-            hi = h2 > hi ? h2 : hi;
-        }
-        if (lane == 0) { red_lo[w] = lo; red_hi[w] = hi; }
-    }
-    __syncthreads();
-    int64_t first = red_lo[0], last = red_hi[0];
-    for (int i = 1; i < 4; i++) {
-        first = red_lo[i] < first ? red_lo[i] : first;
-        last = red_hi[i] > last ? red_hi[i] : last;
-    }
-    Frag<P> qf[KC], df[KC];
-#pragma unroll
-    for (int kc = 0; kc < KC; kc++) {
-        In t1[8], t2[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            t1[j] = x < a.tq ? Q[x * D + kc * 32 + g * 8 + j] : In(0);
-            t2[j] = x < a.tq ? DO[x * D + kc * 32 + g * 8 + j] : In(0);
-        }
-        make_frag<P>(qf[kc], t1);
-        make_frag<P>(df[kc], t2);
-    }
-    const S mrow = x < a.tq ? (S)a.row_max[b * a.tq + x] * (S)LOG2E : S(0);
-    const S lrow = x < a.tq ? (S)a.row_sum[b * a.tq + x] : S(1);
-    const S drow = x < a.tq ? (S)a.delta[b * a.tq + x] : S(0);
+    (void)lane;
+    const int64_t nkt = (a.tk + BW - 1) / BW;
+    const int64_t nqt = (a.tq + BN - 1) / BN;
+    const int64_t rplane = a.batches * a.tqp * D, tplane = a.batches * D * a.tqp;
     const S scale2 = (S)a.scale * (S)LOG2E;
-    AccV dq[DB];
-#pragma unroll
-    for (int i = 0; i < DB; i++) dq[i] = AccV{0, 0, 0, 0};
 
-    if (first < last) {
-        for (int64_t j0 = first - first % BN; j0 < last; j0 += BN) {
-            const bool any = sp.lo < sp.hi && sp.lo < j0 + BN && sp.hi > j0;
-            if (!__syncthreads_or(any)) continue;
-            for (int idx = tid * 4; idx < BN * D; idx += 256 * 4) {
-                const int r = idx / D, col = idx % D;
-                const int64_t j = j0 + r;
+    for (;;) {
+        const int item = take_item(a.ctr, &slot);
+        if (item >= a.items) break;
+        const int64_t b = item % a.batches;
+        const int64_t kt = a.reverse ? nkt - 1 - item / a.batches : item / a.batches;
+        const int64_t k0 = kt * BW;
+        const In *K = static_cast<const In *>(a.k) + b * a.tk * D;
+        const In *V = static_cast<const In *>(a.v) + b * a.tk * D;
+        const E *QR = static_cast<const E *>(a.qr) + b * a.tqp * D;
+        const E *QT = static_cast<const E *>(a.qt) + b * D * a.tqp;
+        const E *DR = static_cast<const E *>(a.dor) + b * a.tqp * D;
+        const E *DT = static_cast<const E *>(a.dot) + b * D * a.tqp;
+        const float *RM = a.row_max + b * a.tq, *RS = a.row_sum + b * a.tq, *DL = a.delta + b * a.tq;
+
+        int64_t kj[2];
+        Frag<P> kf[2][KC], vf[2][KC];
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const In kv = j < a.tk ? K[j * D + col + i] : In(0);
-                    const In vv = j < a.tk ? V[j * D + col + i] : In(0);
-                    tile_store<P>(ks, RPLANE, r * RP + col + i, kv);
-                    tile_store<P>(ktr, TPLANE, (col + i) * TP + r, kv);
-                    tile_store<P>(vs, RPLANE, r * RP + col + i, vv);
+        for (int kb = 0; kb < 2; kb++) {
+            kj[kb] = k0 + w * 32 + kb * 16 + c;
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) {
+                In t1[8], t2[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    t1[j] = kj[kb] < a.tk ? K[kj[kb] * D + kc * 32 + g * 8 + j] : In(0);
+                    t2[j] = kj[kb] < a.tk ? V[kj[kb] * D + kc * 32 + g * 8 + j] : In(0);
                 }
+                make_frag<P>(kf[kb][kc], t1);
+                make_frag<P>(vf[kb][kc], t2);
+            }
+        }
+        AccV dk[2][DB], dv[2][DB];
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+            for (int i = 0; i < DB; i++) { dk[kb][i] = AccV{0, 0, 0, 0}; dv[kb][i] = AccV{0, 0, 0, 0}; }
+
+        // Under a causal mask alone, query x sees key j when j <= x + offset.
+        int64_t qt_begin = 0;
+        if (a.causal_only) {
+            const int64_t xmin = k0 - a.mask.offset;
+            qt_begin = xmin > 0 ? xmin / BN : 0;
+        }
+        for (int64_t qt = qt_begin; qt < nqt; qt++) {
+            const int64_t q0 = qt * BN;
+            if (tid < BN) {
+                const int64_t xq = q0 + tid;
+                const Span sp = query_span(a.mask, xq);
+                slo[tid] = sp.lo;
+                shi[tid] = sp.hi;
+                sm[tid] = xq < a.tq ? RM[xq] : 0.f;
+                sl[tid] = xq < a.tq ? RS[xq] : 1.f;
+                sd[tid] = xq < a.tq ? DL[xq] : 0.f;
             }
             __syncthreads();
-            // S^T = K Q^T and dP^T = V dO^T: lane holds keys kb*16 + crow(g, i)
-            // for query x.
-            AccV s[4], dp[4];
+            bool any = false;
+            if (tid < BN) {
+                const int64_t kend = k0 + BW < a.tk ? k0 + BW : a.tk;
+                any = slo[tid] < shi[tid] && slo[tid] < kend && shi[tid] > k0;
+            }
+            if (!__syncthreads_or(any)) continue;
+            stage<E>(qs, RP, RPLANE, QR + q0 * D, D, rplane, BN, D, PL);
+            stage<E>(qtr, TP, TPLANE, QT + q0, a.tqp, tplane, D, BN, PL);
+            stage<E>(dos, RP, RPLANE, DR + q0 * D, D, rplane, BN, D, PL);
+            stage<E>(dotr, TP, TPLANE, DT + q0, a.tqp, tplane, D, BN, PL);
+            __syncthreads();
+            // The tile's queries in two 32-query halves (hc): S = Q K^T and
+            // dP = dO V^T for its two 16-query blocks, then P and dS, then
+            // dV += P^T dO and dK += dS^T Q over the half.
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++) {
-                s[kb] = AccV{0, 0, 0, 0};
-                dp[kb] = AccV{0, 0, 0, 0};
+            for (int hc = 0; hc < 2; hc++) {
+                AccV s[2][2], dp[2][2];   // [query block][key block]
+#pragma unroll
+                for (int qq = 0; qq < 2; qq++)
+#pragma unroll
+                    for (int kb = 0; kb < 2; kb++) { s[qq][kb] = AccV{0, 0, 0, 0}; dp[qq][kb] = AccV{0, 0, 0, 0}; }
 #pragma unroll
                 for (int kc = 0; kc < KC; kc++) {
-                    Frag<P> af;
-                    tile_load8<P>(af, ks, RPLANE, (kb * 16 + c) * RP + kc * 32 + g * 8);
-                    mma<P>(s[kb], af, qf[kc]);
-                    tile_load8<P>(af, vs, RPLANE, (kb * 16 + c) * RP + kc * 32 + g * 8);
-                    mma<P>(dp[kb], af, df[kc]);
+#pragma unroll
+                    for (int qq = 0; qq < 2; qq++) {
+                        const int qb = 2 * hc + qq;
+                        Frag<P> af, df;
+                        tile_load8<P>(af, qs, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
+                        tile_load8<P>(df, dos, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
+#pragma unroll
+                        for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                            for (int kb = 0; kb < 2; kb++) {
+                                mma_part<P>(s[qq][kb], af, kf[kb][kc], sp_);
+                                mma_part<P>(dp[qq][kb], df, vf[kb][kc], sp_);
+                            }
+                    }
                 }
-            }
-            S ds[4][4];
+                // P = exp(scale s - m) / l, dS = P (dP - delta), 0 where masked.
+                S p[2][2][4], ds[2][2][4];
 #pragma unroll
-            for (int kb = 0; kb < 4; kb++)
+                for (int qq = 0; qq < 2; qq++)
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int64_t j = j0 + kb * 16 + crow<P>(g, i);
-                    const bool ok = j >= sp.lo && j < sp.hi;
-                    const S pv = ok ? exp2(s[kb][i] * scale2 - mrow) / lrow : S(0);
-                    ds[kb][i] = pv * (dp[kb][i] - drow);
+                    for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
+                            const bool ok = kj[kb] >= slo[r] && kj[kb] < shi[r];
+                            const S pv = ok ? fast_exp2(s[qq][kb][i] * scale2 - (S)sm[r] * (S)LOG2E) / (S)sl[r] : S(0);
+                            p[qq][kb][i] = pv;
+                            ds[qq][kb][i] = pv * (dp[qq][kb][i] - (S)sd[r]);
+                        }
+                Frag<P> pf[2], dsf[2];
+#pragma unroll
+                for (int kb = 0; kb < 2; kb++) {
+                    S t1[8], t2[8];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        t1[i] = p[0][kb][i]; t1[i + 4] = p[1][kb][i];
+                        t2[i] = ds[0][kb][i]; t2[i + 4] = ds[1][kb][i];
+                    }
+                    make_frag<P>(pf[kb], t1);
+                    make_frag<P>(dsf[kb], t2);
                 }
-            // dQ^T += K^T dS^T: A = K^T (d x key, from the transposed tile),
-            // B = dS^T (key x query) from registers.
-#pragma unroll
-            for (int kc = 0; kc < 2; kc++) {
-                Frag<P> dsf;
-                S t[8];
-#pragma unroll
-                for (int i = 0; i < 4; i++) { t[i] = ds[2 * kc][i]; t[i + 4] = ds[2 * kc + 1][i]; }
-                make_frag<P>(dsf, t);
 #pragma unroll
                 for (int db = 0; db < DB; db++) {
-                    Frag<P> kt;
-                    tile_load_cperm<P>(kt, ktr, TPLANE, (db * 16 + c) * TP + kc * 32, g);
-                    mma<P>(dq[db], kt, dsf);
+                    Frag<P> bo, bq;
+                    const int row = (db * 16 + c) * TP + hc * 32;
+                    tile_load_cperm<P>(bo, dotr, TPLANE, row, g);
+                    tile_load_cperm<P>(bq, qtr, TPLANE, row, g);
+#pragma unroll
+                    for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                        for (int kb = 0; kb < 2; kb++) {
+                            mma_part<P>(dv[kb][db], pf[kb], bo, sp_);
+                            mma_part<P>(dk[kb][db], dsf[kb], bq, sp_);
+                        }
                 }
             }
             __syncthreads();
         }
+        // Lane holds dK/dV[key k0 + w*32 + kb*16 + crow(g, i)][d = db*16 + c].
+        In *DK = static_cast<In *>(a.dk) + b * a.tk * D;
+        In *DV = static_cast<In *>(a.dv) + b * a.tk * D;
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t key = k0 + w * 32 + kb * 16 + crow<P>(g, i);
+                if (key < a.tk) {
+#pragma unroll
+                    for (int db = 0; db < DB; db++) {
+                        DK[key * D + db * 16 + c] = (In)(dk[kb][db][i] * (S)a.scale);
+                        DV[key * D + db * 16 + c] = (In)dv[kb][db][i];
+                    }
+                }
+            }
     }
-    // Lane holds dQ^T[d = db*16 + crow(g, i)][query x].
-    if (x < a.tq) {
-        In *DQ = static_cast<In *>(a.dq) + b * a.tq * D + x * D;
+}
+
+// dQ for 128 queries per workgroup (32 per wave), visiting the key tiles
+// each may see.
+template <int P, int D>
+__global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
+    typedef typename Input<P>::T In;
+    typedef typename Acc<P>::type AccV;
+    typedef typename Acc<P>::scalar S;
+    typedef typename Elem<P>::T E;
+    constexpr int PL = Elem<P>::planes;
+    constexpr int KC = D / 32, DB = D / 16;
+    constexpr int RP = D + 8, TP = BN + 8;
+    constexpr int RPLANE = BN * RP, TPLANE = D * TP;
+    __shared__ __attribute__((aligned(16))) E ks[PL * RPLANE];
+    __shared__ __attribute__((aligned(16))) E ktr[PL * TPLANE];
+    __shared__ __attribute__((aligned(16))) E vs[PL * RPLANE];
+    __shared__ int64_t red_lo[4], red_hi[4];
+    __shared__ int slot;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
+    (void)lane;
+    const int64_t nqt = (a.tq + BW - 1) / BW;
+    const int64_t rplane = a.batches * a.tkp * D, tplane = a.batches * D * a.tkp;
+    const S scale2 = (S)a.scale * (S)LOG2E;
+
+    for (;;) {
+        const int item = take_item(a.ctr, &slot);
+        if (item >= a.items) break;
+        const int64_t b = item % a.batches;
+        const int64_t qt = a.reverse ? nqt - 1 - item / a.batches : item / a.batches;
+        const int64_t q0 = qt * BW;
+        const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
+        const In *DO = static_cast<const In *>(a.dout) + b * a.tq * D;
+        const E *KR = static_cast<const E *>(a.kr) + b * a.tkp * D;
+        const E *KT = static_cast<const E *>(a.kt) + b * D * a.tkp;
+        const E *VR = static_cast<const E *>(a.vr) + b * a.tkp * D;
+
+        int64_t x[2];
+        Span sp[2];
+        Frag<P> qf[2][KC], df[2][KC];
+        S mrow[2], lrow[2], drow[2];
 #pragma unroll
-        for (int db = 0; db < DB; db++)
+        for (int qb = 0; qb < 2; qb++) {
+            x[qb] = q0 + w * 32 + qb * 16 + c;
+            sp[qb] = query_span(a.mask, x[qb]);
 #pragma unroll
-            for (int i = 0; i < 4; i++) DQ[db * 16 + crow<P>(g, i)] = (In)(dq[db][i] * (S)a.scale);
+            for (int kc = 0; kc < KC; kc++) {
+                In t1[8], t2[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    t1[j] = x[qb] < a.tq ? Q[x[qb] * D + kc * 32 + g * 8 + j] : In(0);
+                    t2[j] = x[qb] < a.tq ? DO[x[qb] * D + kc * 32 + g * 8 + j] : In(0);
+                }
+                make_frag<P>(qf[qb][kc], t1);
+                make_frag<P>(df[qb][kc], t2);
+            }
+            mrow[qb] = x[qb] < a.tq ? (S)a.row_max[b * a.tq + x[qb]] * (S)LOG2E : S(0);
+            lrow[qb] = x[qb] < a.tq ? (S)a.row_sum[b * a.tq + x[qb]] : S(1);
+            drow[qb] = x[qb] < a.tq ? (S)a.delta[b * a.tq + x[qb]] : S(0);
+        }
+        int64_t first, last;
+        group_range(sp, red_lo, red_hi, first, last);
+        AccV dq[2][DB];
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+            for (int i = 0; i < DB; i++) dq[qb][i] = AccV{0, 0, 0, 0};
+
+        for (int64_t j0 = first < last ? first - first % BN : last; j0 < last; j0 += BN) {
+            stage<E>(ks, RP, RPLANE, KR + j0 * D, D, rplane, BN, D, PL);
+            stage<E>(ktr, TP, TPLANE, KT + j0, a.tkp, tplane, D, BN, PL);
+            stage<E>(vs, RP, RPLANE, VR + j0 * D, D, rplane, BN, D, PL);
+            __syncthreads();
+            // The tile's keys in two 32-key halves: S^T = K Q^T and
+            // dP^T = V dO^T, then dS^T, then dQ^T += K^T dS^T.
+#pragma unroll
+            for (int hc = 0; hc < 2; hc++) {
+                AccV s[2][2], dp[2][2];   // [key block][query block]
+#pragma unroll
+                for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+                    for (int qb = 0; qb < 2; qb++) { s[kk][qb] = AccV{0, 0, 0, 0}; dp[kk][qb] = AccV{0, 0, 0, 0}; }
+#pragma unroll
+                for (int kc = 0; kc < KC; kc++) {
+#pragma unroll
+                    for (int kk = 0; kk < 2; kk++) {
+                        const int kb = 2 * hc + kk;
+                        Frag<P> af, vf;
+                        tile_load8<P>(af, ks, RPLANE, (kb * 16 + c) * RP + kc * 32 + g * 8);
+                        tile_load8<P>(vf, vs, RPLANE, (kb * 16 + c) * RP + kc * 32 + g * 8);
+#pragma unroll
+                        for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                            for (int qb = 0; qb < 2; qb++) {
+                                mma_part<P>(s[kk][qb], af, qf[qb][kc], sp_);
+                                mma_part<P>(dp[kk][qb], vf, df[qb][kc], sp_);
+                            }
+                    }
+                }
+                Frag<P> dsf[2];
+#pragma unroll
+                for (int qb = 0; qb < 2; qb++) {
+                    S t[8];
+#pragma unroll
+                    for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int64_t j = j0 + (2 * hc + kk) * 16 + crow<P>(g, i);
+                            const bool ok = j >= sp[qb].lo && j < sp[qb].hi;
+                            const S pv = ok ? fast_exp2(s[kk][qb][i] * scale2 - mrow[qb]) / lrow[qb] : S(0);
+                            t[kk * 4 + i] = pv * (dp[kk][qb][i] - drow[qb]);
+                        }
+                    make_frag<P>(dsf[qb], t);
+                }
+                Frag<P> kt[DB];
+#pragma unroll
+                for (int db = 0; db < DB; db++) tile_load_cperm<P>(kt[db], ktr, TPLANE, (db * 16 + c) * TP + hc * 32, g);
+#pragma unroll
+                for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                    for (int db = 0; db < DB; db++)
+#pragma unroll
+                        for (int qb = 0; qb < 2; qb++) mma_part<P>(dq[qb][db], kt[db], dsf[qb], sp_);
+            }
+            __syncthreads();
+        }
+        // Lane holds dQ^T[d = db*16 + crow(g, i)][query x].
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++) {
+            if (x[qb] >= a.tq) continue;
+            In *DQ = static_cast<In *>(a.dq) + (b * a.tq + x[qb]) * D;
+#pragma unroll
+            for (int db = 0; db < DB; db++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) DQ[db * 16 + crow<P>(g, i)] = (In)(dq[qb][db][i] * (S)a.scale);
+        }
     }
 }
 
@@ -620,132 +849,248 @@ Mask make_mask(const ef_attention_desc &d) {
     return m;
 }
 
-template <int P, int D>
-void launch_fwd(const FwdArgs &a, int64_t batches, hipStream_t s) {
-    dim3 grid((unsigned)((a.tq + BM - 1) / BM), (unsigned)batches);
-    hipLaunchKernelGGL((attn_fwd_kernel<P, D>), grid, dim3(256), 0, s, a);
+int compute_units() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            cus = 256;
+    }
+    return cus;
 }
 
-template <int P>
-bool dispatch_fwd(const FwdArgs &a, int64_t batches, int64_t d, hipStream_t s) {
-    switch (d) {
-    case 32: launch_fwd<P, 32>(a, batches, s); return true;
-    case 64: launch_fwd<P, 64>(a, batches, s); return true;
-    case 128:
-        if constexpr (P == PREC_F64) return false;   // LDS: the generic kernel takes it
-        else { launch_fwd<P, 128>(a, batches, s); return true; }
-    default: return false;
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+bool fast_fwd_shape(const ef_attention_desc &d) {
+    if (d.features != d.values) return false;
+    // The kernel's 32-bit offsets into a plane of prepared operands.
+    if (2 * d.batches * (size_t)round_up((int64_t)d.keys, BW) * d.features >= (size_t(1) << 31)) return false;
+    if (d.precision == EF_PRECISION_FP64) return d.features == 32 || d.features == 64;
+    return d.features == 32 || d.features == 64 || d.features == 128;
+}
+
+bool fast_bwd_shape(const ef_attention_desc &d) {
+    return d.features == d.values && (d.features == 32 || d.features == 64);
+}
+
+size_t elem_bytes(uint32_t precision) {
+    switch (precision) {
+    case EF_PRECISION_FP64: return 8;
+    case EF_PRECISION_FP32: return 4;
+    case EF_PRECISION_BF16X3: return 4;   // two BF16 planes
+    default: return 2;
     }
 }
 
+// Scratch layout, each region 256-byte aligned: the work counters, then
+// the forward's prepared K and V^T, or (from the same offset) the
+// backward's prepared tiles and each query's delta.
+struct Layout {
+    size_t counters, kr, vt;
+    size_t qr, qt, dor, dot, bkr, bkt, bvr, delta;
+    size_t total;
+};
+
+Layout layout(const ef_attention_desc &d) {
+    Layout L;
+    const size_t e = elem_bytes(d.precision);
+    const size_t tqp = (size_t)round_up((int64_t)d.queries, BW), tkp = (size_t)round_up((int64_t)d.keys, BW);
+    const size_t qa = align256(d.batches * tqp * d.features * e), ka = align256(d.batches * tkp * d.features * e);
+    size_t at = 0;
+    L.counters = at; at += 256;
+    L.kr = at; at += ka;
+    L.vt = at; at += ka;
+    const size_t fwd_end = at;
+    at = 256;
+    L.qr = at; at += qa;
+    L.qt = at; at += qa;
+    L.dor = at; at += qa;
+    L.dot = at; at += qa;
+    L.bkr = at; at += ka;
+    L.bkt = at; at += ka;
+    L.bvr = at; at += ka;
+    L.delta = at; at += align256(d.batches * d.queries * 4);
+    L.total = at > fwd_end ? at : fwd_end;
+    return L;
+}
+
 template <int P, int D>
-void launch_bwd(const BwdArgs &a, int64_t batches, int stage, hipStream_t s) {
+ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, void *out,
+                  float *row_max, float *row_sum, uint64_t *visited, char *scratch, hipStream_t s) {
     typedef typename Input<P>::T In;
-    const int64_t rows = batches * a.tq;
-    const int64_t nqt = (a.tq + BM - 1) / BM;
-    if (stage & 1)
-        hipLaunchKernelGGL((attn_delta_kernel<In, D>), dim3((unsigned)((rows + 15) / 16)), dim3(256), 0, s, a, rows);
-    if (stage & 2) {
-        dim3 g1((unsigned)((a.tk + BM - 1) / BM), (unsigned)batches);
-        hipLaunchKernelGGL((attn_bwd_dkdv_kernel<P, D>), g1, dim3(256), 0, s, a, (int64_t)0, nqt);
-    }
-    if (stage & 4) {
-        dim3 g2((unsigned)nqt, (unsigned)batches);
-        hipLaunchKernelGGL((attn_bwd_dq_kernel<P, D>), g2, dim3(256), 0, s, a);
-    }
+    typedef typename Elem<P>::T E;
+    const Layout L = layout(d);
+    const int64_t B = (int64_t)d.batches, tq = (int64_t)d.queries, tk = (int64_t)d.keys;
+    const int64_t tkp = round_up(tk, BW);
+    E *kr = reinterpret_cast<E *>(scratch + L.kr);
+    E *vt = reinterpret_cast<E *>(scratch + L.vt);
+    dim3 pg((unsigned)(tkp / 64), (unsigned)B);
+    hipLaunchKernelGGL((prep_kernel<P, D>), pg, dim3(256), 0, s, (const In *)k, tk, tkp, B, kr, (E *)nullptr);
+    hipLaunchKernelGGL((prep_kernel<P, D>), pg, dim3(256), 0, s, (const In *)v, tk, tkp, B, (E *)nullptr, vt);
+    FwdArgs a;
+    a.q = q; a.kr = kr; a.vt = vt; a.out = out;
+    a.row_max = row_max; a.row_sum = row_sum;
+    a.visited = reinterpret_cast<unsigned long long *>(visited);
+    a.ctr = reinterpret_cast<Counter *>(scratch + L.counters);
+    a.mask = make_mask(d);
+    a.tq = tq; a.tk = tk; a.tkp = tkp; a.batches = B;
+    a.scale = d.scale;
+    a.items = (int)(B * ((tq + BW - 1) / BW));
+    a.reverse = d.causal ? 1 : 0;
+    const int cap = 2 * compute_units();
+    const int grid = a.items < cap ? a.items : cap;
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
+    hipLaunchKernelGGL((attn_fwd_kernel<P, D>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    EF_HIP(hipGetLastError());
+    return EF_OK;
 }
 
 template <int P>
-bool dispatch_bwd(const BwdArgs &a, int64_t batches, const ef_attention_desc &desc, int stage, hipStream_t s) {
-    switch (desc.features) {
-    case 32: launch_bwd<P, 32>(a, batches, stage, s); return true;
-    case 64: launch_bwd<P, 64>(a, batches, stage, s); return true;
-    default: return false;
+ef_status dispatch_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, void *out,
+                       float *m, float *l, uint64_t *vis, char *scratch, hipStream_t s) {
+    switch (d.features) {
+    case 32: return run_fwd<P, 32>(d, q, k, v, out, m, l, vis, scratch, s);
+    case 64: return run_fwd<P, 64>(d, q, k, v, out, m, l, vis, scratch, s);
+    case 128:
+        if constexpr (P != PREC_F64) return run_fwd<P, 128>(d, q, k, v, out, m, l, vis, scratch, s);
+        [[fallthrough]];
+    default:
+        set_error("no fast attention kernel for this head size");
+        return EF_INVALID;
+    }
+}
+
+template <int P, int D>
+ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, const void *out,
+                  const void *dout, const float *row_max, const float *row_sum, void *dq, void *dk, void *dv,
+                  char *scratch, int stages, hipStream_t s) {
+    typedef typename Input<P>::T In;
+    typedef typename Elem<P>::T E;
+    const Layout L = layout(d);
+    const int64_t B = (int64_t)d.batches, tq = (int64_t)d.queries, tk = (int64_t)d.keys;
+    const int64_t tqp = round_up(tq, BW), tkp = round_up(tk, BW);
+    BwdArgs a;
+    a.q = q; a.k = k; a.v = v; a.dout = dout;
+    a.qr = scratch + L.qr; a.qt = scratch + L.qt; a.dor = scratch + L.dor; a.dot = scratch + L.dot;
+    a.kr = scratch + L.bkr; a.kt = scratch + L.bkt; a.vr = scratch + L.bvr;
+    a.row_max = row_max; a.row_sum = row_sum;
+    a.delta = reinterpret_cast<float *>(scratch + L.delta);
+    a.dq = dq; a.dk = dk; a.dv = dv;
+    a.mask = make_mask(d);
+    a.tq = tq; a.tk = tk; a.tqp = tqp; a.tkp = tkp; a.batches = B;
+    a.scale = d.scale;
+    a.causal_only = (!d.allowed && d.causal && d.window <= 0) ? 1 : 0;
+    a.pad = 0;
+    if (stages & 1) {
+        dim3 gq((unsigned)(tqp / 64), (unsigned)B), gk((unsigned)(tkp / 64), (unsigned)B);
+        hipLaunchKernelGGL((prep_kernel<P, D>), gq, dim3(256), 0, s, (const In *)q, tq, tqp, B, (E *)a.qr, (E *)a.qt);
+        hipLaunchKernelGGL((prep_kernel<P, D>), gq, dim3(256), 0, s, (const In *)dout, tq, tqp, B, (E *)a.dor,
+                           (E *)a.dot);
+        hipLaunchKernelGGL((prep_kernel<P, D>), gk, dim3(256), 0, s, (const In *)k, tk, tkp, B, (E *)a.kr, (E *)a.kt);
+        hipLaunchKernelGGL((prep_kernel<P, D>), gk, dim3(256), 0, s, (const In *)v, tk, tkp, B, (E *)a.vr,
+                           (E *)nullptr);
+        const int64_t rows = B * tq;
+        hipLaunchKernelGGL((attn_delta_kernel<In, D>), dim3((unsigned)((rows + 15) / 16)), dim3(256), 0, s,
+                           (const In *)out, (const In *)dout, (float *)a.delta, rows);
+    }
+    const int cap = compute_units();
+    if (stages & 2) {
+        a.ctr = reinterpret_cast<Counter *>(scratch + L.counters);
+        a.items = (int)(B * ((tk + BW - 1) / BW));
+        a.reverse = 0;   // key tiles in order: under a causal mask the first are the heaviest
+        const int grid = a.items < cap ? a.items : cap;
+        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
+        hipLaunchKernelGGL((attn_bwd_dkdv_kernel<P, D>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    }
+    if (stages & 4) {
+        a.ctr = reinterpret_cast<Counter *>(scratch + L.counters) + 1;
+        a.items = (int)(B * ((tq + BW - 1) / BW));
+        a.reverse = d.causal ? 1 : 0;   // under a causal mask the last query tiles are the heaviest
+        const int grid = a.items < cap ? a.items : cap;
+        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
+        hipLaunchKernelGGL((attn_bwd_dq_kernel<P, D>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    }
+    EF_HIP(hipGetLastError());
+    return EF_OK;
+}
+
+template <int P>
+ef_status dispatch_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, const void *out,
+                       const void *dout, const float *m, const float *l, void *dq, void *dk, void *dv, char *scratch,
+                       int stages, hipStream_t s) {
+    switch (d.features) {
+    case 32: return run_bwd<P, 32>(d, q, k, v, out, dout, m, l, dq, dk, dv, scratch, stages, s);
+    case 64: return run_bwd<P, 64>(d, q, k, v, out, dout, m, l, dq, dk, dv, scratch, stages, s);
+    default:
+        set_error("the attention backward pass takes 32 or 64 features, equal for keys and values");
+        return EF_INVALID;
     }
 }
 
 }  // namespace
 
-ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
-                        void *out, float *row_max, float *row_sum, uint64_t *tiles_visited,
-                        hipStream_t s) {
+size_t attention_scratch_bytes(const ef_attention_desc &d) { return layout(d).total; }
+
+ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, void *out,
+                        float *row_max, float *row_sum, uint64_t *tiles_visited, void *scratch, hipStream_t s) {
     if (d.batches == 0 || d.queries == 0) return EF_OK;
-    FwdArgs a;
-    a.q = q; a.k = k; a.v = v; a.out = out;
-    a.row_max = row_max; a.row_sum = row_sum;
-    a.visited = reinterpret_cast<unsigned long long *>(tiles_visited);
-    a.mask = make_mask(d);
-    a.tq = (int64_t)d.queries;
-    a.tk = (int64_t)d.keys;
-    a.scale = d.scale;
-    a.reverse = d.causal ? 1 : 0;
-    const bool fast = d.features == d.values && (d.features == 32 || d.features == 64 || d.features == 128);
-    bool ok = false;
-    if (fast) {
+    if (fast_fwd_shape(d) && d.precision <= EF_PRECISION_BF16) {
+        if (!scratch) {
+            set_error("the attention forward pass needs its scratch");
+            return EF_INVALID;
+        }
+        char *sc = static_cast<char *>(scratch);
         switch (d.precision) {
-        case EF_PRECISION_FP32: ok = dispatch_fwd<PREC_F32>(a, d.batches, d.features, s); break;
-        case EF_PRECISION_BF16X3: ok = dispatch_fwd<PREC_BF16X3>(a, d.batches, d.features, s); break;
-        case EF_PRECISION_BF16: ok = dispatch_fwd<PREC_BF16>(a, d.batches, d.features, s); break;
-        case EF_PRECISION_FP64: ok = dispatch_fwd<PREC_F64>(a, d.batches, d.features, s); break;
-        default: break;
+        case EF_PRECISION_FP32: return dispatch_fwd<PREC_F32>(d, q, k, v, out, row_max, row_sum, tiles_visited, sc, s);
+        case EF_PRECISION_BF16X3: return dispatch_fwd<PREC_BF16X3>(d, q, k, v, out, row_max, row_sum, tiles_visited, sc, s);
+        case EF_PRECISION_BF16: return dispatch_fwd<PREC_BF16>(d, q, k, v, out, row_max, row_sum, tiles_visited, sc, s);
+        default: return dispatch_fwd<PREC_F64>(d, q, k, v, out, row_max, row_sum, tiles_visited, sc, s);
         }
     }
-    if (!ok) {
-        // Any other shape: the generic kernel (FP64 for double tensors,
-        // FP32 for float ones; the tile count is not tracked here).
-        const int64_t chunks = ((int64_t)d.values + 7) / 8;
-        const size_t threads = d.batches * d.queries * (size_t)chunks;
-        if (d.precision == EF_PRECISION_FP64)
-            hipLaunchKernelGGL(attn_fwd_generic<double>, dim3(blocks_for(threads, 256)), dim3(256), 0, s,
-                               (const double *)q, (const double *)k, (const double *)v, (double *)out,
-                               row_max, row_sum, a.mask, (int64_t)d.batches, a.tq, a.tk,
-                               (int64_t)d.features, (int64_t)d.values, d.scale);
-        else
-            hipLaunchKernelGGL(attn_fwd_generic<float>, dim3(blocks_for(threads, 256)), dim3(256), 0, s,
-                               (const float *)q, (const float *)k, (const float *)v, (float *)out,
-                               row_max, row_sum, a.mask, (int64_t)d.batches, a.tq, a.tk,
-                               (int64_t)d.features, (int64_t)d.values, d.scale);
-    }
+    // Any other shape: the generic kernel (FP64 for double tensors, FP32
+    // for float ones; no tile count).
+    const Mask mask = make_mask(d);
+    const int64_t chunks = ((int64_t)d.values + 7) / 8;
+    const size_t threads = d.batches * d.queries * (size_t)chunks;
+    if (d.precision == EF_PRECISION_FP64)
+        hipLaunchKernelGGL(attn_fwd_generic<double>, dim3(blocks_for(threads, 256)), dim3(256), 0, s,
+                           (const double *)q, (const double *)k, (const double *)v, (double *)out, row_max, row_sum,
+                           mask, (int64_t)d.batches, (int64_t)d.queries, (int64_t)d.keys, (int64_t)d.features,
+                           (int64_t)d.values, d.scale);
+    else
+        hipLaunchKernelGGL(attn_fwd_generic<float>, dim3(blocks_for(threads, 256)), dim3(256), 0, s,
+                           (const float *)q, (const float *)k, (const float *)v, (float *)out, row_max, row_sum,
+                           mask, (int64_t)d.batches, (int64_t)d.queries, (int64_t)d.keys, (int64_t)d.features,
+                           (int64_t)d.values, d.scale);
     EF_HIP(hipGetLastError());
     return EF_OK;
 }
 
-size_t attention_bwd_scratch_bytes(const ef_attention_desc &d) {
-    return d.batches * d.queries * sizeof(float);
-}
-
-ef_status attention_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
-                        const void *out, const void *dout, const float *row_max,
-                        const float *row_sum, void *dq, void *dk, void *dv, void *scratch,
-                        int stages, hipStream_t s) {
+ef_status attention_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, const void *out,
+                        const void *dout, const float *row_max, const float *row_sum, void *dq, void *dk, void *dv,
+                        void *scratch, int stages, hipStream_t s) {
     if (d.batches == 0) return EF_OK;
-    if (!(d.features == d.values && (d.features == 32 || d.features == 64))) {
+    if (!fast_bwd_shape(d)) {
         set_error("the attention backward pass takes 32 or 64 features, equal for keys and values");
         return EF_INVALID;
     }
-    BwdArgs a;
-    a.q = q; a.k = k; a.v = v; a.out = out; a.dout = dout;
-    a.row_max = row_max; a.row_sum = row_sum;
-    a.delta = static_cast<float *>(scratch);
-    a.dq = dq; a.dk = dk; a.dv = dv;
-    a.mask = make_mask(d);
-    a.tq = (int64_t)d.queries;
-    a.tk = (int64_t)d.keys;
-    a.scale = d.scale;
-    bool ok = false;
-    switch (d.precision) {
-    case EF_PRECISION_FP32: ok = dispatch_bwd<PREC_F32>(a, d.batches, d, stages, s); break;
-    case EF_PRECISION_BF16X3: ok = dispatch_bwd<PREC_BF16X3>(a, d.batches, d, stages, s); break;
-    case EF_PRECISION_BF16: ok = dispatch_bwd<PREC_BF16>(a, d.batches, d, stages, s); break;
-    case EF_PRECISION_FP64: ok = dispatch_bwd<PREC_F64>(a, d.batches, d, stages, s); break;
-    default: break;
+    if (!scratch) {
+        set_error("the attention backward pass needs its scratch");
+        return EF_INVALID;
     }
-    if (!ok) {
+    char *sc = static_cast<char *>(scratch);
+    switch (d.precision) {
+    case EF_PRECISION_FP32: return dispatch_bwd<PREC_F32>(d, q, k, v, out, dout, row_max, row_sum, dq, dk, dv, sc, stages, s);
+    case EF_PRECISION_BF16X3: return dispatch_bwd<PREC_BF16X3>(d, q, k, v, out, dout, row_max, row_sum, dq, dk, dv, sc, stages, s);
+    case EF_PRECISION_BF16: return dispatch_bwd<PREC_BF16>(d, q, k, v, out, dout, row_max, row_sum, dq, dk, dv, sc, stages, s);
+    case EF_PRECISION_FP64: return dispatch_bwd<PREC_F64>(d, q, k, v, out, dout, row_max, row_sum, dq, dk, dv, sc, stages, s);
+    default:
         set_error("unknown precision");
         return EF_INVALID;
     }
-    EF_HIP(hipGetLastError());
-    return EF_OK;
 }
 
 }  // namespace ef

@@ -69,13 +69,13 @@ ef_status launch_dense_product_f64(const double *a, const double *b, double *out
 ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *first,
                    size_t *n_ranks, void *scratch, hipStream_t s);
 size_t rank_scratch_bytes(size_t n);
+size_t attention_scratch_bytes(const ef_attention_desc &d);
 ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
                         void *out, float *row_max, float *row_sum, uint64_t *tiles_visited,
-                        hipStream_t s);
-size_t attention_bwd_scratch_bytes(const ef_attention_desc &d);
+                        void *scratch, hipStream_t s);
 ef_status attention_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
                         const void *out, const void *dout, const float *row_max,
                         const float *row_sum, void *dq, void *dk, void *dv, void *scratch,
-                        int stages, hipStream_t s);   // stages: 1 delta, 2 dK/dV, 4 dQ
+                        int stages, hipStream_t s);   // stages: 1 prepare, 2 dK/dV, 4 dQ
 
 }  // namespace ef

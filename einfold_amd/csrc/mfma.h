@@ -81,6 +81,33 @@ template <> __device__ __forceinline__ void mma<PREC_BF16X3>(f32x4 &c, const Fra
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
 }
 
+// One product as separately issued sub-instructions, so that a caller can
+// interleave independent accumulators between them: BF16x3 has three
+// (lo*hi, hi*lo, hi*hi), BF16 one, FP32 and FP64 eight k-steps.
+template <int P> struct Parts { static constexpr int n = 8; };
+template <> struct Parts<PREC_BF16X3> { static constexpr int n = 3; };
+template <> struct Parts<PREC_BF16> { static constexpr int n = 1; };
+
+template <int P>
+__device__ __forceinline__ void mma_part(typename Acc<P>::type &c, const Frag<P> &a, const Frag<P> &b, int s) {
+    if constexpr (P == PREC_F32) {
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v[s], b.v[s], c, 0, 0, 0);
+    } else if constexpr (P == PREC_F64) {
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a.v[s], b.v[s], c, 0, 0, 0);
+    } else if constexpr (P == PREC_BF16) {
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+    } else {
+        if (s == 0) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+        else if (s == 1) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+        else c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
+    }
+}
+
+// 2^x: the hardware's v_exp_f32 for float (no denormal results: a weight
+// below 2^-126 becomes 0), the library's exp2 for double.
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double fast_exp2(double x) { return exp2(x); }
+
 // A fragment from 8 values in registers (split for BF16 precisions).
 template <int P, class T> __device__ __forceinline__ void make_frag(Frag<P> &f, const T (&x)[8]) {
     if constexpr (P == PREC_F32) {

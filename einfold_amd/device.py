@@ -97,10 +97,15 @@ def _desc(b, tq, tk, d, dv, scale, precision, causal, offset, window, allowed):
     return desc
 
 
+def attention_scratch(desc, device) -> torch.Tensor:
+    need = lib().ef_attention_scratch_bytes(ctypes.byref(desc))
+    return torch.empty(max(need, 1), dtype=torch.uint8, device=device)
+
+
 def attention_fwd(q: torch.Tensor, k: torch.Tensor, v: torch.Tensor, scale: float,
                   causal: bool = False, precision: str = "bf16x3", offset: int = 0,
                   window: int = 0, allowed: Optional[torch.Tensor] = None,
-                  count_tiles: bool = False):
+                  count_tiles: bool = False, scratch: Optional[torch.Tensor] = None):
     """Fused softmax attention over [batch][position][feature] tensors.
     Returns (out, row_max, row_sum[, tiles_visited])."""
     dtype = torch.float64 if precision == "fp64" else torch.float32
@@ -117,8 +122,10 @@ def attention_fwd(q: torch.Tensor, k: torch.Tensor, v: torch.Tensor, scale: floa
     l = torch.empty((b, tq), dtype=torch.float32, device=q.device)
     visited = torch.zeros(1, dtype=torch.int64, device=q.device) if count_tiles else None
     desc = _desc(b, tq, tk, d, dv, scale, precision, causal, offset, window, allowed)
+    if scratch is None or scratch.numel() < lib().ef_attention_scratch_bytes(ctypes.byref(desc)):
+        scratch = attention_scratch(desc, q.device)
     check(lib().ef_attention_fwd(ctypes.byref(desc), _ptr(q), _ptr(k), _ptr(v), _ptr(out),
-                                 _ptr(m), _ptr(l), _ptr(visited), _stream()))
+                                 _ptr(m), _ptr(l), _ptr(visited), _ptr(scratch), _stream()))
     if count_tiles:
         return out, m, l, int(visited.item())
     return out, m, l
@@ -140,9 +147,8 @@ def attention_bwd(q, k, v, out, dout, row_max, row_sum, scale: float, causal: bo
     _need(row_max, torch.float32, (b, tq), "row_max")
     _need(row_sum, torch.float32, (b, tq), "row_sum")
     desc = _desc(b, tq, tk, d, dv_, scale, precision, causal, offset, window, allowed)
-    need = lib().ef_attention_bwd_scratch_bytes(ctypes.byref(desc))
-    if scratch is None or scratch.numel() < need:
-        scratch = torch.empty(max(need, 1), dtype=torch.uint8, device=q.device)
+    if scratch is None or scratch.numel() < lib().ef_attention_scratch_bytes(ctypes.byref(desc)):
+        scratch = attention_scratch(desc, q.device)
     dq = torch.empty_like(q)
     dk = torch.empty_like(k)
     dvv = torch.empty_like(v)

@@ -306,33 +306,37 @@ typedef struct ef_attention_desc {
     uint32_t pad;
 } ef_attention_desc;
 
+/* Device scratch for the attention entry points below (the operands
+ * prepared in MFMA fragment order, work counters and per-row sums): one
+ * buffer of this many bytes serves the forward and backward passes of
+ * `desc`.  Calls that share a buffer must run in order (one stream). */
+size_t ef_attention_scratch_bytes(const ef_attention_desc *desc);
+
 ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const void *k,
                            const void *v, void *out, float *row_max, float *row_sum,
-                           uint64_t *tiles_visited, void *stream);
+                           uint64_t *tiles_visited, void *scratch, void *stream);
 
 /* The backward pass of ef_attention_fwd, recomputing score tiles from the
  * saved row_max/row_sum (never storing the T x T scores): given dout
- * (the upstream gradient), writes dq, dk, dv.  `scratch` is device memory
- * of ef_attention_bwd_scratch_bytes(desc) bytes. */
-size_t ef_attention_bwd_scratch_bytes(const ef_attention_desc *desc);
+ * (the upstream gradient), writes dq, dk, dv.  32 or 64 features. */
 ef_status ef_attention_bwd(const ef_attention_desc *desc, const void *q, const void *k,
                            const void *v, const void *out, const void *dout,
                            const float *row_max, const float *row_sum,
                            void *dq, void *dk, void *dv, void *scratch, void *stream);
 
 /* ef_attention_bwd in its three stages, for callers that want only some
- * gradients (a frozen query table needs no dq): prepare writes each
- * query's sum of dout * out into scratch; bwd_kv then gives dk and dv,
- * bwd_q gives dq, each from that scratch. */
-ef_status ef_attention_bwd_prepare(const ef_attention_desc *desc, const void *out, const void *dout,
+ * gradients (a frozen query table needs no dq): prepare fills the scratch
+ * (the operands in fragment order and each query's sum of dout * out);
+ * bwd_kv then gives dk and dv, bwd_q gives dq, each from that scratch. */
+ef_status ef_attention_bwd_prepare(const ef_attention_desc *desc, const void *q, const void *k,
+                                   const void *v, const void *out, const void *dout,
                                    void *scratch, void *stream);
-ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *q, const void *k,
-                              const void *v, const void *dout, const float *row_max,
-                              const float *row_sum, const void *scratch, void *dk, void *dv,
-                              void *stream);
-ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const void *k,
-                             const void *v, const void *dout, const float *row_max,
-                             const float *row_sum, const void *scratch, void *dq, void *stream);
+ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *k, const void *v,
+                              const float *row_max, const float *row_sum, void *scratch,
+                              void *dk, void *dv, void *stream);
+ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const void *dout,
+                             const float *row_max, const float *row_sum, void *scratch,
+                             void *dq, void *stream);
 
 #ifdef __cplusplus
 }
