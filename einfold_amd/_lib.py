@@ -128,6 +128,7 @@ def lib():
                 f.argtypes = [ctypes.POINTER(desc), ctypes.POINTER(CRelation), ctypes.POINTER(CRelation),
                               ctypes.POINTER(CRelation), ctypes.POINTER(COptions), ctypes.POINTER(CResult)]
                 f.restype = st
+        if hasattr(L, "ef_pipeline"):
             L.ef_pipeline.argtypes = [ctypes.POINTER(ir.CPipeline), ctypes.POINTER(CRelation),
                                       ctypes.POINTER(CRelation), ctypes.POINTER(CRelation),
                                       ctypes.POINTER(COptions), ctypes.POINTER(CResult)]

@@ -69,6 +69,10 @@ ef_status launch_dense_product_f64(const double *a, const double *b, double *out
 ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *first,
                    size_t *n_ranks, void *scratch, hipStream_t s);
 size_t rank_scratch_bytes(size_t n);
+// Stable sort of (word, row) pairs by word; rows_in null means 0..n-1.
+// `scratch` is rank_scratch_bytes(n) bytes; w_out may be null.
+ef_status sort_u64(const uint64_t *words, const uint32_t *rows_in, size_t n, uint64_t *w_out, uint32_t *r_out,
+                   void *scratch, hipStream_t s);
 size_t attention_scratch_bytes(const ef_attention_desc &d);
 ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
                         void *out, float *row_max, float *row_sum, uint64_t *tiles_visited,

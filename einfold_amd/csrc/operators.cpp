@@ -1,0 +1,1176 @@
+// The kernel API that EinFoldExec calls (crates/einfold-kernels/src/
+// algorithm.rs:52-160): fold descriptions and Arrow batches in, Arrow
+// batches out, with the device doing the per-row work.  This file holds the
+// shared host pieces and the dense positional fold (dense.rs:176-650).
+#include "operators.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "discriminate.h"
+#include "ops.h"
+
+namespace ef {
+
+namespace {
+
+constexpr uint64_t U32_MAX = 0xffffffffULL;
+constexpr uint64_t FEW_CELLS = 4096;          // dense.rs:85-91
+constexpr uint64_t SATURATED = ~0ULL;
+
+// Saturating products of extents (a size past 2^64 fails every check).
+uint64_t mul_sat(uint64_t a, uint64_t b) {
+    const unsigned __int128 p = (unsigned __int128)a * b;
+    return p > SATURATED ? SATURATED : (uint64_t)p;
+}
+
+uint64_t size_of(const std::vector<uint32_t> &dims, const std::vector<uint64_t> &extent) {
+    uint64_t s = 1;
+    for (uint32_t d : dims) s = mul_sat(s, extent[d]);
+    return s;
+}
+
+// dense.rs:85-91: a matrix is worth it with at most four cells per input
+// row (plus slack for small inputs), and positions must fit 32 bits.
+bool worth(uint64_t cells, uint64_t rows) {
+    return cells <= mul_sat(rows, 4) + FEW_CELLS && cells < U32_MAX;
+}
+
+int index_in(const std::vector<uint32_t> &v, uint32_t d) {
+    for (size_t i = 0; i < v.size(); i++)
+        if (v[i] == d) return (int)i;
+    return -1;
+}
+
+// The stride of dimension d within its role (later dimensions vary faster).
+uint64_t stride_in(const std::vector<uint32_t> &role, uint32_t d, const std::vector<uint64_t> &extent) {
+    uint64_t s = 1;
+    for (size_t i = (size_t)index_in(role, d) + 1; i < role.size(); i++) s *= extent[role[i]];
+    return s;
+}
+
+bool factor_type_ok(Type t, bool accept_float32) {
+    return t == Type::Float64 || (accept_float32 && t == Type::Float32);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// Shared pieces
+// ---------------------------------------------------------------------
+
+bool parse_fold(const ef_fold_desc *f, FoldModel &m) {
+    m = FoldModel();
+    if (!f) {
+        set_error("no fold description");
+        return false;
+    }
+    const uint32_t nd = f->n_dimensions;
+    if (nd && !f->equality) {
+        set_error("a fold's dimensions need their key equalities");
+        return false;
+    }
+    m.equality.assign(f->equality, f->equality + nd);
+    for (uint32_t e : m.equality)
+        if (e > EF_NOT_DISTINCT) {
+            set_error("unknown key equality");
+            return false;
+        }
+    if (f->n_operands && !f->operands) {
+        set_error("operands missing");
+        return false;
+    }
+    for (uint32_t o = 0; o < f->n_operands; o++) {
+        const ef_operand &op = f->operands[o];
+        OperandModel om;
+        om.factor = op.factor;
+        for (uint32_t k = 0; k < op.n_keys; k++) {
+            if (op.keys[k].dimension >= nd) {
+                set_error("a key names a dimension the fold doesn't have");
+                return false;
+            }
+            om.keys.emplace_back(op.keys[k].dimension, op.keys[k].column);
+        }
+        m.operands.push_back(om);
+    }
+    for (uint32_t i = 0; i < f->n_outputs; i++) {
+        if (f->outputs[i].dimension >= nd) {
+            set_error("an output names a dimension the fold doesn't have");
+            return false;
+        }
+        m.outputs.push_back(f->outputs[i]);
+    }
+    for (uint32_t i = 0; i < f->n_conditions; i++) {
+        const ef_condition &c = f->conditions[i];
+        if (c.left >= nd || c.right >= nd || c.comparison > EF_GREATER_OR_EQUAL) {
+            set_error("a condition names a dimension the fold doesn't have");
+            return false;
+        }
+        m.conditions.push_back(c);
+    }
+    if (f->domain) {
+        m.has_domain = true;
+        m.carry.assign(f->domain->carry, f->domain->carry + f->domain->n_carry);
+        m.has_missing = f->domain->has_missing != 0;
+        m.missing = f->domain->missing;
+        for (auto &o : m.outputs)
+            if (o.domain_column < 0) {
+                set_error("with a domain every output needs its domain column");
+                return false;
+            }
+    }
+    if (f->row && f->n_row) m.row.assign(f->row, f->row + f->n_row);
+    return true;
+}
+
+bool fold_schema(const FoldModel &m, const std::vector<const Relation *> &operands, const Relation *domain,
+                 bool accept_float32, std::vector<OutField> &fields) {
+    fields.clear();
+    if (m.operands.size() != operands.size()) {
+        set_error("one relation per operand");
+        return false;
+    }
+    if (m.has_domain != (domain != nullptr)) {
+        set_error("a domain relation must be given exactly when the fold has a domain");
+        return false;
+    }
+    const size_t nd = m.equality.size();
+    std::vector<int> have(nd, 0);
+    std::vector<OutField> dim(nd, OutField{Type::Other, true});
+    for (size_t o = 0; o < m.operands.size(); o++) {
+        const OperandModel &op = m.operands[o];
+        const Relation &r = *operands[o];
+        if (m.row.empty()) {
+            if (op.factor < 0) {
+                set_error("EinFold doesn't compute masks yet");
+                return false;
+            }
+            if ((size_t)op.factor >= r.columns() || !factor_type_ok(r.types[op.factor], accept_float32)) {
+                set_error("factors must be Float64");
+                return false;
+            }
+        }
+        for (auto &k : op.keys) {
+            if (k.second >= r.columns()) {
+                set_error("a key's column is past its relation's columns");
+                return false;
+            }
+            const OutField f{r.types[k.second], (bool)r.nullable[k.second]};
+            if (!have[k.first]) {
+                have[k.first] = 1;
+                dim[k.first] = f;
+            } else if (dim[k.first].type != f.type) {
+                set_error("dimension " + std::to_string(k.first) + "'s columns have two types");
+                return false;
+            }
+        }
+    }
+    for (const ef_expr_op &e : m.row) {
+        if (e.code != EF_EXPR_COLUMN) continue;
+        if (e.operand >= operands.size() || e.column >= operands[e.operand]->columns() ||
+            !factor_type_ok(operands[e.operand]->types[e.column], accept_float32)) {
+            set_error("an expression's columns must be Float64");
+            return false;
+        }
+    }
+    if (!m.has_domain) {
+        for (auto &o : m.outputs) {
+            if (!have[o.dimension]) {
+                set_error("an output's dimension is in no operand");
+                return false;
+            }
+            fields.push_back(dim[o.dimension]);
+        }
+        fields.push_back(OutField{Type::Float64, true});
+    } else {
+        for (uint32_t c : m.carry) {
+            if (c >= domain->columns()) {
+                set_error("a carried column is past the domain's columns");
+                return false;
+            }
+            fields.push_back(OutField{domain->types[c], (bool)domain->nullable[c]});
+        }
+        fields.push_back(OutField{Type::Float64, !m.has_missing});
+    }
+    return true;
+}
+
+ef_status Call::reserve(double bytes, const char *what) {
+    if (!options || !options->budget_grow) return EF_OK;
+    const size_t b = bytes >= 1.8e19 ? std::numeric_limits<size_t>::max() : (size_t)bytes;
+    if (options->budget_grow(options->budget_ctx, b)) return EF_OK;
+    set_error("EinFold's budget can't give " + std::to_string(b) + " bytes for " + what);
+    return EF_OUT_OF_MEMORY;
+}
+
+namespace {
+
+// Uploads one batch's bitmap bits [offset, offset + n) and unpacks them.
+ef_status unpack_validity(Call &c, const ColumnView &v, uint8_t *dst, uint8_t *&stage, size_t &stage_bytes) {
+    const int64_t first = v.bit_offset >> 3;
+    const size_t bytes = (size_t)(((v.bit_offset + v.length + 7) >> 3) - first);
+    if (bytes > stage_bytes) {
+        EF_TRY(c.alloc(stage, bytes));
+        stage_bytes = bytes;
+    }
+    EF_TRY(c.upload<uint8_t>(stage, v.validity + first, bytes));
+    return unpack_bits(v.length, stage, v.bit_offset & 7, dst, c.stream);
+}
+
+}  // namespace
+
+ef_status upload_validity(Call &c, const Relation &r, int column, uint8_t *valid) {
+    uint8_t *stage = nullptr;
+    size_t stage_bytes = 0;
+    int64_t off = 0;
+    for (size_t b = 0; b < r.batches.size(); b++) {
+        const ColumnView &v = r.batches[b][column];
+        if (v.validity) EF_TRY(unpack_validity(c, v, valid + off, stage, stage_bytes));
+        else if (v.length) EF_HIP(hipMemsetAsync(valid + off, 1, v.length, c.stream));
+        off += v.length;
+    }
+    return EF_OK;
+}
+
+ef_status upload_factor(Call &c, const Relation &r, int column, double *values, uint8_t *valid, bool *any_null) {
+    float *stage32 = nullptr;
+    size_t stage32_n = 0;
+    int64_t off = 0;
+    for (size_t b = 0; b < r.batches.size(); b++) {
+        const ColumnView &v = r.batches[b][column];
+        if (any_null && v.null_count) *any_null = true;
+        if (v.type == Type::Float64) {
+            EF_TRY(c.upload<double>(values + off, v.values, v.length));
+        } else if (v.type == Type::Float32) {
+            if ((size_t)v.length > stage32_n) {
+                EF_TRY(c.alloc(stage32, v.length));
+                stage32_n = v.length;
+            }
+            EF_TRY(c.upload<float>(stage32, v.values, v.length));
+            EF_TRY(to_f64(Type::Float32, stage32, v.length, values + off, c.stream));
+        } else {
+            set_error("factors must be floats");
+            return EF_INVALID;
+        }
+        off += v.length;
+    }
+    return valid ? upload_validity(c, r, column, valid) : EF_OK;
+}
+
+ef_status upload_raw(Call &c, const Relation &r, int column, void *dst) {
+    const int w = width_of(r.types[column]);
+    int64_t off = 0;
+    for (size_t b = 0; b < r.batches.size(); b++) {
+        const ColumnView &v = r.batches[b][column];
+        EF_TRY(c.upload<uint8_t>(static_cast<uint8_t *>(dst) + off * w, v.values, (size_t)v.length * w));
+        off += v.length;
+    }
+    return EF_OK;
+}
+
+OutColumn copy_column(const ColumnView &v, int64_t rows, bool nullable) {
+    OutColumn o;
+    o.type = v.type;
+    o.nullable = nullable;
+    o.length = rows;
+    const int w = width_of(v.type);
+    o.values = std::make_shared<HostBuffer>((size_t)rows * w);
+    if (rows) std::memcpy(o.values->data, v.values, (size_t)rows * w);
+    if (v.validity && v.null_count) {
+        o.validity = std::make_shared<HostBuffer>((size_t)(rows + 7) / 8);
+        std::memset(o.validity->data, 0, (size_t)(rows + 7) / 8);
+        for (int64_t r = 0; r < rows; r++)
+            if (v.valid(r)) o.validity->data[r >> 3] |= (uint8_t)(1u << (r & 7));
+        o.null_count = v.null_count;
+    }
+    return o;
+}
+
+ef_status fetch_column(Call &c, Type t, const void *values, const uint8_t *valid, int64_t n, bool nullable,
+                       OutColumn &out) {
+    out = OutColumn();
+    out.type = t;
+    out.nullable = nullable;
+    out.length = n;
+    const int w = width_of(t);
+    out.values = std::make_shared<HostBuffer>((size_t)n * w);
+    EF_TRY(c.download<uint8_t>(out.values->data, static_cast<const uint8_t *>(values), (size_t)n * w));
+    unsigned long long nulls = 0;
+    std::shared_ptr<HostBuffer> bits;
+    if (valid && n) {
+        uint8_t *dbits;
+        unsigned long long *dnulls;
+        EF_TRY(c.alloc(dbits, (size_t)(n + 7) / 8));
+        EF_TRY(c.zeros(dnulls, 1));
+        EF_TRY(pack_bits(n, valid, dbits, dnulls, c.stream));
+        bits = std::make_shared<HostBuffer>((size_t)(n + 7) / 8);
+        EF_TRY(c.download<uint8_t>(bits->data, dbits, (size_t)(n + 7) / 8));
+        EF_TRY(c.download<unsigned long long>(&nulls, dnulls, 1));
+    }
+    EF_TRY(c.sync());
+    if (nulls) {
+        out.validity = bits;
+        out.null_count = (int64_t)nulls;
+    }
+    return EF_OK;
+}
+
+void put_result(std::vector<std::vector<OutColumn>> batches, const std::vector<OutField> &fields, uint32_t algorithm,
+                ef_result *out) {
+    std::memset(out, 0, sizeof *out);
+    export_schema(fields, &out->schema);
+    out->n_batches = batches.size();
+    out->batches = batches.empty() ? nullptr : new ArrowArray[batches.size()];
+    for (size_t i = 0; i < batches.size(); i++) export_batch(std::move(batches[i]), &out->batches[i]);
+    out->algorithm = algorithm;
+}
+
+// ---------------------------------------------------------------------
+// The dense positional fold (dense.rs:176-578)
+// ---------------------------------------------------------------------
+
+ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation &b, const Relation *domain,
+                          const std::vector<OutField> &fields, Call &c,
+                          std::vector<std::vector<OutColumn>> &result) {
+    const bool accept32 = c.options && c.options->accept_float32;
+    if (m.operands.size() != 2) return EF_DECLINED;
+    const OperandModel &A = m.operands[0], &B = m.operands[1];
+    const uint32_t nd = (uint32_t)m.equality.size();
+    // Roles (dense.rs:196-230): W in both operands and the output, X in a
+    // only, Y in b only (output order); Z aggregated, in both (dimension
+    // order).  A row of the output is W's then X's.
+    std::vector<uint32_t> ws, xs, ys, zs;
+    for (auto &o : m.outputs) {
+        const bool ina = A.column_of(o.dimension) >= 0, inb = B.column_of(o.dimension) >= 0;
+        if (ina && inb) ws.push_back(o.dimension);
+        else if (ina) xs.push_back(o.dimension);
+        else if (inb) ys.push_back(o.dimension);
+        else return EF_DECLINED;
+    }
+    xs.insert(xs.begin(), ws.begin(), ws.end());
+    for (uint32_t d = 0; d < nd; d++) {
+        if (!m.aggregated(d)) continue;
+        if (A.column_of(d) < 0 || B.column_of(d) < 0) return EF_DECLINED;
+        zs.push_back(d);
+    }
+    auto is_out = [&](uint32_t d) { return index_in(xs, d) >= 0 || index_in(ys, d) >= 0; };
+    for (auto &cd : m.conditions)
+        if (!is_out(cd.left) || !is_out(cd.right)) return EF_DECLINED;
+    auto integers = [](const OperandModel &op, const Relation &r) {
+        if (r.batches.empty()) return true;
+        for (auto &k : op.keys)
+            if (!is_integer(r.types[k.second])) return false;
+        return true;
+    };
+    if (!integers(A, a) || !integers(B, b)) return EF_DECLINED;
+    EF_TRY(c.reserve(((double)a.rows * A.keys.size() + (double)b.rows * B.keys.size()) * 8,
+                     "the dense algorithm's keys"));
+    // dense.rs:97-111: keys without NULLs, non-negative, below 2^63 - 1.
+    for (auto *pr : {&a, &b}) {
+        const OperandModel &op = pr == &a ? A : B;
+        for (auto &bt : pr->batches)
+            for (auto &k : op.keys)
+                if (bt[k.second].null_count) return EF_DECLINED;
+    }
+    KeyStats *stats;
+    EF_TRY(c.alloc(stats, nd ? nd : 1));
+    {
+        std::vector<KeyStats> init(nd ? nd : 1, KeyStats{0, -1, 0, 0});
+        EF_TRY(c.upload<KeyStats>(stats, init.data(), init.size()));
+    }
+    // Each operand's keys as i64 per dimension, every batch in one array.
+    std::vector<int64_t *> a_key(nd, nullptr), b_key(nd, nullptr);
+    size_t raw_bytes = 0;
+    for (auto &k : A.keys) raw_bytes = std::max(raw_bytes, (size_t)a.rows * width_of(a.types[k.second]));
+    for (auto &k : B.keys) raw_bytes = std::max(raw_bytes, (size_t)b.rows * width_of(b.types[k.second]));
+    uint8_t *raw = nullptr;
+    EF_TRY(c.alloc(raw, raw_bytes));
+    for (auto *pr : {&a, &b}) {
+        const OperandModel &op = pr == &a ? A : B;
+        auto &dst = pr == &a ? a_key : b_key;
+        for (auto &k : op.keys) {
+            EF_TRY(c.alloc(dst[k.first], (size_t)pr->rows));
+            EF_TRY(upload_raw(c, *pr, (int)k.second, raw));
+            EF_TRY(cast_keys(pr->types[k.second], raw, pr->rows, dst[k.first], stats + k.first, c.stream));
+        }
+    }
+    std::vector<KeyStats> st(nd ? nd : 1);
+    EF_TRY(c.download<KeyStats>(st.data(), stats, st.size()));
+    EF_TRY(c.sync());
+    std::vector<uint64_t> extent(nd, 0);
+    for (uint32_t d = 0; d < nd; d++) {
+        if (st[d].bad || st[d].lo < 0 || st[d].hi == std::numeric_limits<int64_t>::max()) return EF_DECLINED;
+        extent[d] = (uint64_t)(st[d].hi + 1);
+    }
+    std::vector<uint32_t> wzs = ws;
+    wzs.insert(wzs.end(), zs.begin(), zs.end());
+    const uint64_t nw = size_of(ws, extent), nx = size_of(xs, extent), nz = size_of(zs, extent),
+                   ny = size_of(ys, extent);
+    const uint64_t per_batch = nw ? nx / nw : 0;
+    const uint64_t out_rows = domain ? (uint64_t)domain->rows : mul_sat(nx, ny);
+    if (!worth(mul_sat(nx, nz), a.rows) || !worth(mul_sat(mul_sat(nw, nz), ny), b.rows) ||
+        !worth(mul_sat(nx, ny), out_rows))
+        return EF_DECLINED;
+    const uint64_t na = nx * nz, nb = nw * nz * ny, ncell = nx * ny;
+    const uint64_t out_columns = m.outputs.size() + 1 + (domain ? 1 : 0);
+    const double working = (double)na * 9 + (double)nx * 2 + (double)nb * 9 + ((double)a.rows + b.rows) * 8 +
+                           (double)ncell * 16 + (double)out_rows * out_columns * 8;
+    EF_TRY(c.reserve(working, "the dense algorithm's matrices"));
+
+    uint64_t missing_bits;
+    std::memcpy(&missing_bits, &m.missing, 8);
+    const bool zero_default = m.has_domain && m.has_missing && missing_bits == 0;
+    for (auto *pr : {&a, &b}) {
+        const OperandModel &op = pr == &a ? A : B;
+        if (pr->batches.empty()) continue;
+        if (op.factor < 0 || !factor_type_ok(pr->types[op.factor], accept32)) return EF_DECLINED;
+    }
+    auto dims = [&](const std::vector<uint32_t> &role, const std::vector<int64_t *> &keys) {
+        Dims d{};
+        d.n = (int)role.size();
+        for (size_t i = 0; i < role.size(); i++) {
+            d.key[i] = keys[role[i]];
+            d.extent[i] = (int64_t)extent[role[i]];
+        }
+        return d;
+    };
+    if (xs.size() > MAX_DIMS || wzs.size() > MAX_DIMS || ys.size() > MAX_DIMS || zs.size() > MAX_DIMS ||
+        m.outputs.size() > MAX_DIMS || m.conditions.size() > MAX_CONDS) {
+        set_error("more dimensions or conditions than the device kernels take");
+        return EF_DECLINED;
+    }
+    // Placement (dense.rs:319-393).
+    double *am, *bm, *fa, *fb;
+    uint32_t *held, *present;
+    uint8_t *reached, *valued = nullptr, *va, *vb;
+    PlaceFlags *flags;
+    EF_TRY(c.zeros(am, na));
+    EF_TRY(c.zeros(held, na));
+    EF_TRY(c.zeros(reached, nx));
+    if (!zero_default) EF_TRY(c.zeros(valued, nx));
+    EF_TRY(c.zeros(bm, nb));
+    EF_TRY(c.zeros(present, nb));
+    EF_TRY(c.zeros(flags, 1));
+    EF_TRY(c.alloc(fa, (size_t)a.rows));
+    EF_TRY(c.alloc(va, (size_t)a.rows));
+    EF_TRY(c.alloc(fb, (size_t)b.rows));
+    EF_TRY(c.alloc(vb, (size_t)b.rows));
+    bool b_nulls = false;
+    if (a.rows) EF_TRY(upload_factor(c, a, A.factor, fa, va, nullptr));
+    if (b.rows) EF_TRY(upload_factor(c, b, B.factor, fb, vb, &b_nulls));
+    EF_TRY(place_a(a.rows, dims(xs, a_key), dims(zs, a_key), nz, fa, va, held, am, reached, valued, flags,
+                   c.stream));
+    EF_TRY(place_b(b.rows, dims(wzs, b_key), dims(ys, b_key), ny, fb, vb, present, bm, flags, c.stream));
+    PlaceFlags pf;
+    EF_TRY(c.download<PlaceFlags>(&pf, flags, 1));
+    EF_TRY(c.sync());
+    if (pf.duplicate) return EF_DECLINED;
+    const bool finite = !pf.not_finite;
+    // Without duplicates, b is complete when it has one row per cell.
+    const bool holes = b_nulls || (uint64_t)b.rows != nb;
+    if (holes && (!zero_default || !finite)) return EF_DECLINED;
+
+    std::vector<uint32_t> order = xs;
+    order.insert(order.end(), ys.begin(), ys.end());
+    Conds conds{};
+    conds.n = (int)m.conditions.size();
+    for (size_t i = 0; i < m.conditions.size(); i++) {
+        const ef_condition &cd = m.conditions[i];
+        conds.op[i] = (int)cd.comparison;
+        conds.left_in_x[i] = index_in(xs, cd.left) >= 0;
+        conds.right_in_x[i] = index_in(xs, cd.right) >= 0;
+        conds.left_stride[i] = stride_in(conds.left_in_x[i] ? xs : ys, cd.left, extent);
+        conds.right_stride[i] = stride_in(conds.right_in_x[i] ? xs : ys, cd.right, extent);
+        conds.left_extent[i] = extent[cd.left] ? extent[cd.left] : 1;
+        conds.right_extent[i] = extent[cd.right] ? extent[cd.right] : 1;
+        conds.offset[i] = cd.offset;
+    }
+    // The domain's rows located on cells (dense.rs:400-434, 595-615).
+    std::vector<int64_t *> cells;
+    if (domain) {
+        CellKeys ck{};
+        ck.n = (int)m.outputs.size();
+        int64_t *dkeys;
+        uint8_t *dvalid;
+        const size_t nrow = (size_t)domain->rows;
+        EF_TRY(c.alloc(dkeys, nrow * m.outputs.size()));
+        EF_TRY(c.alloc(dvalid, nrow * m.outputs.size()));
+        size_t raw2 = 0;
+        for (auto &o : m.outputs) {
+            const int col = o.domain_column;
+            if (col < 0 || (size_t)col >= domain->columns()) {
+                set_error("a domain column is past the domain's columns");
+                return EF_INVALID;
+            }
+            if (!is_integer(domain->types[col]) && !domain->batches.empty()) return EF_DECLINED;
+            raw2 = std::max(raw2, nrow * width_of(domain->types[col]));
+        }
+        uint8_t *rawd;
+        EF_TRY(c.alloc(rawd, raw2));
+        for (size_t i = 0; i < m.outputs.size(); i++) {
+            const ef_output &o = m.outputs[i];
+            EF_TRY(upload_validity(c, *domain, o.domain_column, dvalid + i * nrow));
+            if (!domain->batches.empty()) {
+                EF_TRY(upload_raw(c, *domain, o.domain_column, rawd));
+                EF_TRY(widen_keys(domain->types[o.domain_column], (int64_t)nrow, rawd, dvalid + i * nrow,
+                                  dkeys + i * nrow, c.stream));
+            }
+            ck.key[i] = dkeys + i * nrow;
+            ck.valid[i] = dvalid + i * nrow;
+            ck.extent[i] = (int64_t)extent[o.dimension];
+            ck.stride[i] = (int64_t)stride_in(order, o.dimension, extent);
+        }
+        int64_t *cell_all;
+        EF_TRY(c.alloc(cell_all, nrow));
+        EF_TRY(locate((int64_t)nrow, ck, cell_all, c.stream));
+        if (c.options && c.options->domain_rows == EF_DOMAIN_ROWS_UNIQUE_KEYS) {
+            uint32_t *seen;
+            int *repeated;
+            EF_TRY(c.zeros(seen, ncell));
+            EF_TRY(c.zeros(repeated, 1));
+            EF_TRY(unique_cells((int64_t)nrow, cell_all, seen, repeated, c.stream));
+            int rep = 0;
+            EF_TRY(c.download<int>(&rep, repeated, 1));
+            EF_TRY(c.sync());
+            if (rep) return EF_DECLINED;
+        }
+        int64_t off = 0;
+        for (size_t bi = 0; bi < domain->batches.size(); bi++) {
+            cells.push_back(cell_all + off);
+            off += domain->batch_rows[bi];
+        }
+    }
+
+    // The contraction (dense.rs:706-770), in the reference's order.
+    double *out;
+    EF_TRY(c.alloc(out, ncell));
+    if (ncell) EF_TRY(launch_dense_accumulate_f64(am, held, bm, nx, nz, ny, per_batch ? per_batch : 1, 1, out,
+                                                  c.stream));
+
+    if (domain) {
+        // dense.rs:488-515: each domain row gets its cell's value where the
+        // group exists (and has a value), else the default or NULL.
+        for (size_t bi = 0; bi < domain->batches.size(); bi++) {
+            const int64_t n = domain->batch_rows[bi];
+            double *vals;
+            uint8_t *valid;
+            EF_TRY(c.alloc(vals, (size_t)n));
+            EF_TRY(c.alloc(valid, (size_t)n));
+            EF_TRY(domain_values(n, cells[bi], ny, out, zero_default ? nullptr : valued, conds, m.has_missing,
+                                 m.missing, vals, valid, c.stream));
+            std::vector<OutColumn> cols;
+            for (size_t i = 0; i < m.carry.size(); i++)
+                cols.push_back(copy_column(domain->batches[bi][m.carry[i]], n, fields[i].nullable));
+            OutColumn v;
+            EF_TRY(fetch_column(c, Type::Float64, vals, m.has_missing ? nullptr : valid, n, fields.back().nullable, v));
+            cols.push_back(std::move(v));
+            result.push_back(std::move(cols));
+        }
+        return EF_OK;
+    }
+    // dense.rs:516-577: every cell of every reached row, keys recomputed
+    // from positions, filtered by the conditions.
+    uint32_t *tmp32, *excl, *rows;
+    void *scan_tmp;
+    uint8_t *scan_bytes;
+    EF_TRY(c.alloc(tmp32, nx));
+    EF_TRY(c.alloc(excl, nx));
+    EF_TRY(c.alloc(rows, nx));
+    EF_TRY(c.alloc(scan_bytes, scan_scratch_bytes(std::max<uint64_t>(nx, ncell))));
+    scan_tmp = scan_bytes;
+    EF_TRY(reached_rows((int64_t)nx, reached, tmp32, excl, rows, scan_tmp, c.stream));
+    uint32_t last[2] = {0, 0};
+    uint8_t last_reached = 0;
+    if (nx) {
+        EF_TRY(c.download<uint32_t>(&last[0], excl + nx - 1, 1));
+        EF_TRY(c.download<uint8_t>(&last_reached, reached + nx - 1, 1));
+    }
+    EF_TRY(c.sync());
+    const uint64_t n_rows = nx ? (uint64_t)last[0] + last_reached : 0;
+    const uint64_t n = n_rows * ny;
+    uint32_t *keep = nullptr, *kexcl = nullptr;
+    uint64_t kept = n;
+    if (!m.conditions.empty() && n) {
+        EF_TRY(c.alloc(keep, n));
+        EF_TRY(c.alloc(kexcl, n));
+        EF_TRY(keep_cells((int64_t)n, ny, rows, conds, keep, c.stream));
+        EF_TRY(scan_u32(keep, kexcl, n, scan_tmp, c.stream));
+        uint32_t lk[2];
+        EF_TRY(c.download<uint32_t>(&lk[0], kexcl + n - 1, 1));
+        EF_TRY(c.download<uint32_t>(&lk[1], keep + n - 1, 1));
+        EF_TRY(c.sync());
+        kept = (uint64_t)lk[0] + lk[1];
+    }
+    OutKeys ok{};
+    ok.n = (int)m.outputs.size();
+    std::vector<uint8_t *> key_out(m.outputs.size());
+    for (size_t i = 0; i < m.outputs.size(); i++) {
+        const uint32_t d = m.outputs[i].dimension;
+        const int w = width_of(fields[i].type);
+        EF_TRY(c.alloc(key_out[i], kept * (w ? w : 8)));
+        ok.dst[i] = key_out[i];
+        ok.width[i] = w;
+        ok.in_x[i] = index_in(xs, d) >= 0;
+        ok.stride[i] = stride_in(ok.in_x[i] ? xs : ys, d, extent);
+        ok.extent[i] = extent[d] ? extent[d] : 1;
+    }
+    double *vals;
+    uint8_t *valid;
+    EF_TRY(c.alloc(vals, kept));
+    EF_TRY(c.alloc(valid, kept));
+    EF_TRY(emit((int64_t)n, ny, rows, keep, kexcl, ok, out, valued, vals, valid, c.stream));
+    std::vector<OutColumn> cols;
+    for (size_t i = 0; i < m.outputs.size(); i++) {
+        OutColumn k;
+        EF_TRY(fetch_column(c, fields[i].type, key_out[i], nullptr, (int64_t)kept, fields[i].nullable, k));
+        cols.push_back(std::move(k));
+    }
+    OutColumn v;
+    EF_TRY(fetch_column(c, Type::Float64, vals, valid, (int64_t)kept, true, v));
+    cols.push_back(std::move(v));
+    result.push_back(std::move(cols));
+    return EF_OK;
+}
+
+// ---------------------------------------------------------------------
+// The discrimination algorithm (discriminate.rs:240-650)
+// ---------------------------------------------------------------------
+
+namespace {
+
+// A key column on the device: the rows of one or more relations' columns
+// concatenated, as raw values and a validity byte per row.
+struct DevColumn {
+    Type type = Type::Other;
+    uint8_t *raw = nullptr;
+    uint8_t *valid = nullptr;
+    int64_t n = 0;
+    bool nulls = false;
+};
+
+ef_status upload_column(Call &c, const std::vector<std::pair<const Relation *, int>> &parts, bool zero_signs_equal,
+                        DevColumn &out) {
+    out = DevColumn();
+    out.type = parts[0].first->types[parts[0].second];
+    for (auto &p : parts) {
+        if (p.first->types[p.second] != out.type) {
+            set_error("a dimension's key columns have two types");
+            return EF_INVALID;
+        }
+        out.n += p.first->rows;
+        for (auto &bt : p.first->batches)
+            if (bt[p.second].null_count) out.nulls = true;
+    }
+    const int w = width_of(out.type);
+    if (!w) {
+        set_error("keys must be integers or floats");
+        return EF_INVALID;
+    }
+    EF_TRY(c.alloc(out.raw, (size_t)out.n * w));
+    EF_TRY(c.alloc(out.valid, (size_t)out.n));
+    int64_t off = 0;
+    for (auto &p : parts) {
+        EF_TRY(upload_raw(c, *p.first, p.second, out.raw + off * w));
+        EF_TRY(upload_validity(c, *p.first, p.second, out.valid + off));
+        off += p.first->rows;
+    }
+    if (zero_signs_equal && is_float(out.type)) EF_TRY(zero_signs(out.type, out.raw, out.n, c.stream));
+    return EF_OK;
+}
+
+// rank_columns (discriminate.rs:240-277): each row's dense rank in the
+// key order of `cols` (first most significant, NULL first), and each
+// rank's lowest row.  A column is ranked by its order words; a nullable
+// one again by 0 for NULL, 2r + 1 otherwise; a later column joins the
+// running rank as (rank << 32 | its rank), ranked again.
+ef_status rank_columns(Call &c, const std::vector<DevColumn *> &cols, int64_t n, uint32_t *rank, uint32_t *first,
+                       size_t &k, void *scratch) {
+    k = 0;
+    if (n == 0) return EF_OK;
+    if (cols.empty()) {
+        EF_HIP(hipMemsetAsync(rank, 0, (size_t)n * 4, c.stream));
+        EF_HIP(hipMemsetAsync(first, 0, 4, c.stream));
+        k = 1;
+        return EF_OK;
+    }
+    uint64_t *w;
+    uint32_t *r;
+    EF_TRY(c.alloc(w, (size_t)n));
+    EF_TRY(c.alloc(r, (size_t)n));
+    for (size_t ci = 0; ci < cols.size(); ci++) {
+        const DevColumn &col = *cols[ci];
+        uint32_t *dst = ci == 0 ? rank : r;
+        EF_TRY(order_words(col.type, col.raw, col.nulls ? col.valid : nullptr, n, w, c.stream));
+        EF_TRY(rank_u64(w, (size_t)n, dst, first, &k, scratch, c.stream));
+        if (col.nulls) {
+            EF_TRY(null_codes(dst, col.valid, n, w, c.stream));
+            EF_TRY(rank_u64(w, (size_t)n, dst, first, &k, scratch, c.stream));
+        }
+        if (ci > 0) {
+            EF_TRY(combine_ranks(rank, r, n, w, c.stream));
+            EF_TRY(rank_u64(w, (size_t)n, rank, first, &k, scratch, c.stream));
+        }
+    }
+    return EF_OK;
+}
+
+// An operand's joining rows in (hi, lo, factor) order (discriminate.rs:
+// 493-506): stable sorts by the pair of ranks, and where a pair repeats,
+// first by the factor's bits and NULL first (break_ties), all stable from
+// row order.  Returns the sorted words (hi << 32 | lo) and rows.
+ef_status sort_rows(Call &c, const uint32_t *rows, int64_t n, const uint32_t *hi, const uint32_t *lo,
+                    const uint8_t *valid, const double *value, uint64_t *w_sorted, uint32_t *r_sorted,
+                    void *scratch) {
+    if (n == 0) return EF_OK;
+    uint64_t *w;
+    EF_TRY(c.alloc(w, (size_t)n));
+    EF_TRY(pair_words(rows, n, hi, lo, w, c.stream));
+    EF_TRY(sort_u64(w, rows, (size_t)n, w_sorted, r_sorted, scratch, c.stream));
+    int *flag;
+    EF_TRY(c.alloc(flag, 1));
+    bool repeats = false;
+    EF_TRY(any_repeated(w_sorted, n, flag, &repeats, c.stream));
+    if (!repeats) return EF_OK;
+    uint64_t *bits, *vw;
+    uint32_t *t1, *t2;
+    EF_TRY(c.alloc(bits, (size_t)n));
+    EF_TRY(c.alloc(vw, (size_t)n));
+    EF_TRY(c.alloc(t1, (size_t)n));
+    EF_TRY(c.alloc(t2, (size_t)n));
+    EF_TRY(tie_words(rows, n, valid, value, bits, vw, c.stream));
+    EF_TRY(sort_u64(bits, rows, (size_t)n, nullptr, t1, scratch, c.stream));
+    EF_TRY(tie_words(t1, n, valid, value, bits, vw, c.stream));
+    EF_TRY(sort_u64(vw, t1, (size_t)n, nullptr, t2, scratch, c.stream));
+    EF_TRY(pair_words(t2, n, hi, lo, w, c.stream));
+    return sort_u64(w, t2, (size_t)n, w_sorted, r_sorted, scratch, c.stream);
+}
+
+uint64_t column_bytes(const Relation &r, int col) {
+    uint64_t b = (uint64_t)r.rows * width_of(r.types[col]);
+    for (auto &bt : r.batches)
+        if (bt[col].validity) b += (uint64_t)(bt[col].length + 7) / 8;
+    return b;
+}
+
+uint64_t relation_bytes(const Relation &r) {
+    uint64_t b = 0;
+    for (size_t col = 0; col < r.columns(); col++) b += column_bytes(r, (int)col);
+    return b;
+}
+
+}  // namespace
+
+ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Relation &b, const Relation *domain,
+                                 const std::vector<OutField> &fields, Call &c,
+                                 std::vector<std::vector<OutColumn>> &result) {
+    const bool accept32 = c.options && c.options->accept_float32;
+    const bool zeros_equal = c.options && c.options->float_keys == EF_FLOAT_KEYS_SIGNED_ZEROS_EQUAL;
+    const int64_t na = a.rows, nb = b.rows, nd = domain ? domain->rows : 0;
+    const uint64_t ranked_rows = std::max<uint64_t>(na + nb, std::max(na, nb) + nd);
+    if (ranked_rows > U32_MAX) {
+        set_error("EinFold's discrimination algorithm ranks at most 2^32 - 1 rows together");
+        return EF_INVALID;
+    }
+    if (m.operands.size() != 2) {
+        set_error("the discrimination algorithm takes two operands");
+        return EF_INVALID;
+    }
+    const OperandModel &A = m.operands[0], &B = m.operands[1];
+    std::vector<uint32_t> shared;
+    for (uint32_t d = 0; d < m.equality.size(); d++)
+        if (A.column_of(d) >= 0 && B.column_of(d) >= 0) shared.push_back(d);
+    std::vector<size_t> a_out, b_out;
+    for (size_t i = 0; i < m.outputs.size(); i++)
+        (A.column_of(m.outputs[i].dimension) >= 0 ? a_out : b_out).push_back(i);
+    if (A.factor < 0 || (size_t)A.factor >= a.columns() || !factor_type_ok(a.types[A.factor], accept32) ||
+        B.factor < 0 || (size_t)B.factor >= b.columns() || !factor_type_ok(b.types[B.factor], accept32)) {
+        set_error("factors must be Float64");
+        return EF_INVALID;
+    }
+    uint64_t key_bytes = 0;
+    for (auto &k : A.keys) key_bytes += column_bytes(a, (int)k.second);
+    for (auto &k : B.keys) key_bytes += column_bytes(b, (int)k.second);
+    const double ranked = ((double)na + nb) * 2 + (double)nd * 2;
+    EF_TRY(c.reserve(ranked * 28 + ((double)na + nb) * 40 + (double)key_bytes * 2,
+                     "the discrimination algorithm's ranks"));
+
+    void *rscratch;
+    uint8_t *rs_bytes;
+    EF_TRY(c.alloc(rs_bytes, rank_scratch_bytes(std::max<uint64_t>(ranked_rows, 1))));
+    rscratch = rs_bytes;
+    double *fa, *fb;
+    uint8_t *va, *vb;
+    EF_TRY(c.alloc(fa, (size_t)na));
+    EF_TRY(c.alloc(va, (size_t)na));
+    EF_TRY(c.alloc(fb, (size_t)nb));
+    EF_TRY(c.alloc(vb, (size_t)nb));
+    if (na) EF_TRY(upload_factor(c, a, A.factor, fa, va, nullptr));
+    if (nb) EF_TRY(upload_factor(c, b, B.factor, fb, vb, nullptr));
+
+    // The join's keys over both operands; rows with a NULL in a key
+    // compared with `=` join nothing.
+    std::deque<DevColumn> zcols;
+    std::vector<DevColumn *> zptr;
+    uint8_t *a_join, *b_join;
+    EF_TRY(c.alloc(a_join, (size_t)na));
+    EF_TRY(c.alloc(b_join, (size_t)nb));
+    if (na) EF_HIP(hipMemsetAsync(a_join, 1, (size_t)na, c.stream));
+    if (nb) EF_HIP(hipMemsetAsync(b_join, 1, (size_t)nb, c.stream));
+    for (uint32_t d : shared) {
+        zcols.emplace_back();
+        EF_TRY(upload_column(c, {{&a, A.column_of(d)}, {&b, B.column_of(d)}}, zeros_equal, zcols.back()));
+        zptr.push_back(&zcols.back());
+        if (m.equality[d] == EF_EQUAL && zcols.back().nulls) {
+            EF_TRY(and_valid(a_join, zcols.back().valid, na, c.stream));
+            EF_TRY(and_valid(b_join, zcols.back().valid + na, nb, c.stream));
+        }
+    }
+    uint32_t *z_rank, *z_first;
+    EF_TRY(c.alloc(z_rank, (size_t)(na + nb)));
+    EF_TRY(c.alloc(z_first, (size_t)(na + nb)));
+    size_t kz = 0;
+    EF_TRY(rank_columns(c, zptr, na + nb, z_rank, z_first, kz, rscratch));
+
+    // The outputs' keys, each operand's with the domain's.
+    std::deque<DevColumn> xcols, ycols;
+    std::vector<DevColumn *> xptr, yptr;
+    for (int side = 0; side < 2; side++) {
+        const auto &outs = side == 0 ? a_out : b_out;
+        const Relation &r = side == 0 ? a : b;
+        const OperandModel &op = side == 0 ? A : B;
+        auto &cols = side == 0 ? xcols : ycols;
+        auto &ptrs = side == 0 ? xptr : yptr;
+        for (size_t i : outs) {
+            std::vector<std::pair<const Relation *, int>> parts = {{&r, op.column_of(m.outputs[i].dimension)}};
+            if (nd) {
+                const int dc = m.outputs[i].domain_column;
+                if (dc < 0 || (size_t)dc >= domain->columns()) {
+                    set_error("a domain column is past the domain's columns");
+                    return EF_INVALID;
+                }
+                parts.push_back({domain, dc});
+            }
+            cols.emplace_back();
+            EF_TRY(upload_column(c, parts, zeros_equal, cols.back()));
+            ptrs.push_back(&cols.back());
+        }
+    }
+    uint32_t *x_rank, *x_first, *y_rank, *y_first;
+    EF_TRY(c.alloc(x_rank, (size_t)(na + nd)));
+    EF_TRY(c.alloc(x_first, (size_t)(na + nd)));
+    EF_TRY(c.alloc(y_rank, (size_t)(nb + nd)));
+    EF_TRY(c.alloc(y_first, (size_t)(nb + nd)));
+    size_t kx = 0, ky = 0;
+    EF_TRY(rank_columns(c, xptr, na + nd, x_rank, x_first, kx, rscratch));
+    EF_TRY(rank_columns(c, yptr, nb + nd, y_rank, y_first, ky, rscratch));
+    const int64_t nx = (int64_t)std::max<size_t>(kx, 1), nz = (int64_t)std::max<size_t>(kz, 1),
+                  ny = (int64_t)std::max<size_t>(ky, 1);
+
+    // a's joining rows in (x, z) order, b's in (z, y) order, duplicates by
+    // their factors.
+    uint32_t *flags32, *excl32, *rows_a, *rows_b;
+    uint8_t *scan_bytes;
+    const int64_t nmax = std::max<int64_t>(std::max(na, nb), 1);
+    EF_TRY(c.alloc(flags32, (size_t)nmax));
+    EF_TRY(c.alloc(excl32, (size_t)nmax));
+    EF_TRY(c.alloc(rows_a, (size_t)na));
+    EF_TRY(c.alloc(rows_b, (size_t)nb));
+    EF_TRY(c.alloc(scan_bytes, scan_scratch_bytes((size_t)nmax)));
+    uint32_t count_a = 0, count_b = 0;
+    EF_TRY(select_rows(a_join, na, flags32, excl32, rows_a, scan_bytes, &count_a, c.stream));
+    EF_TRY(select_rows(b_join, nb, flags32, excl32, rows_b, scan_bytes, &count_b, c.stream));
+    uint64_t *wa, *wb;
+    uint32_t *ra, *rb;
+    EF_TRY(c.alloc(wa, count_a));
+    EF_TRY(c.alloc(ra, count_a));
+    EF_TRY(c.alloc(wb, count_b));
+    EF_TRY(c.alloc(rb, count_b));
+    EF_TRY(sort_rows(c, rows_a, count_a, x_rank, z_rank, va, fa, wa, ra, rscratch));
+    EF_TRY(sort_rows(c, rows_b, count_b, z_rank + na, y_rank, vb, fb, wb, rb, rscratch));
+    uint32_t *a_start, *b_start, *a_z, *b_y;
+    uint8_t *a_valid, *b_valid;
+    double *a_value, *b_value;
+    EF_TRY(c.alloc(a_start, (size_t)nx + 1));
+    EF_TRY(c.alloc(b_start, (size_t)nz + 1));
+    EF_TRY(c.alloc(a_z, count_a));
+    EF_TRY(c.alloc(a_valid, count_a));
+    EF_TRY(c.alloc(a_value, count_a));
+    EF_TRY(c.alloc(b_y, count_b));
+    EF_TRY(c.alloc(b_valid, count_b));
+    EF_TRY(c.alloc(b_value, count_b));
+    EF_TRY(segment_starts(wa, count_a, nx, a_start, c.stream));
+    EF_TRY(segment_starts(wb, count_b, nz, b_start, c.stream));
+    EF_TRY(gather_sorted(ra, wa, count_a, va, fa, a_z, a_valid, a_value, c.stream));
+    EF_TRY(gather_sorted(rb, wb, count_b, vb, fb, b_y, b_valid, b_value, c.stream));
+
+    // The groups: no more than the joined pairs, nor than every (x, y)
+    // (discriminate.rs:530-545), and per x no more than min(ny, its pairs).
+    uint64_t *bound, *bound_off;
+    unsigned long long *pairs_d;
+    EF_TRY(c.alloc(bound, (size_t)nx));
+    EF_TRY(c.alloc(bound_off, (size_t)nx));
+    EF_TRY(c.zeros(pairs_d, 1));
+    EF_TRY(group_bounds(a_start, a_z, b_start, nx, ny, bound, pairs_d, c.stream));
+    uint8_t *scan64;
+    EF_TRY(c.alloc(scan64, scan64_scratch_bytes((size_t)nx)));
+    EF_TRY(scan_u64(bound, bound_off, (size_t)nx, scan64, c.stream));
+    unsigned long long pairs = 0;
+    uint64_t last_off = 0, last_bound = 0;
+    EF_TRY(c.download<unsigned long long>(&pairs, pairs_d, 1));
+    EF_TRY(c.download<uint64_t>(&last_off, bound_off + nx - 1, 1));
+    EF_TRY(c.download<uint64_t>(&last_bound, bound + nx - 1, 1));
+    EF_TRY(c.sync());
+    const double groups = std::min<double>((double)pairs, (double)nx * (double)ny);
+    const double output = m.has_domain ? (double)nd * 8 : groups * 8 * (double)(m.outputs.size() + 1);
+    EF_TRY(c.reserve(groups * 24 + output, "the discrimination algorithm's groups"));
+    const uint64_t staged = last_off + last_bound;
+    uint32_t *g_y, *g_count;
+    double *g_sum;
+    uint8_t *g_valued;
+    EF_TRY(c.alloc(g_y, staged));
+    EF_TRY(c.alloc(g_sum, staged));
+    EF_TRY(c.alloc(g_valued, staged));
+    EF_TRY(c.alloc(g_count, (size_t)nx));
+
+    // Gustavson's product: wave slots, each with an accumulator of ny.
+    size_t free_b = 0, total_b = 0;
+    EF_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double per_slot = (double)ny * 9.0;
+    int64_t slots = std::min<int64_t>(2048, (int64_t)std::max(4.0, std::floor((double)free_b / 4 / per_slot)));
+    slots = std::min<int64_t>(slots, ((nx + 3) / 4) * 4);
+    slots = std::max<int64_t>(4, (slots / 4) * 4);
+    GustavsonArgs g{};
+    g.a_start = a_start;
+    g.a_z = a_z;
+    g.a_valid = a_valid;
+    g.a_value = a_value;
+    g.b_start = b_start;
+    g.b_y = b_y;
+    g.b_valid = b_valid;
+    g.b_value = b_value;
+    g.nx = nx;
+    g.ny = ny;
+    EF_TRY(c.zeros(g.acc, (size_t)(slots * ny)));
+    EF_TRY(c.zeros(g.state, (size_t)(slots * ny)));
+    EF_TRY(c.zeros(g.next, 1));
+    g.bound_off = bound_off;
+    g.g_y = g_y;
+    g.g_sum = g_sum;
+    g.g_valued = g_valued;
+    g.g_count = g_count;
+    EF_TRY(gustavson(g, (int)(slots / 4), c.stream));
+
+    if (!m.has_domain) {
+        uint64_t *counts64, *out_off;
+        EF_TRY(c.alloc(counts64, (size_t)nx));
+        EF_TRY(c.alloc(out_off, (size_t)nx));
+        EF_TRY(widen_counts(g_count, nx, counts64, c.stream));
+        EF_TRY(scan_u64(counts64, out_off, (size_t)nx, scan64, c.stream));
+        uint64_t lo = 0, lc = 0;
+        EF_TRY(c.download<uint64_t>(&lo, out_off + nx - 1, 1));
+        EF_TRY(c.download<uint64_t>(&lc, counts64 + nx - 1, 1));
+        EF_TRY(c.sync());
+        const uint64_t total = lo + lc;
+        GroupKeys keys{};
+        keys.n = (int)m.outputs.size();
+        if (keys.n > 8) {
+            set_error("more than 8 outputs");
+            return EF_INVALID;
+        }
+        std::vector<uint8_t *> kd(keys.n), kv(keys.n);
+        for (size_t i = 0; i < m.outputs.size(); i++) {
+            auto ia = std::find(a_out.begin(), a_out.end(), i);
+            const DevColumn *src = ia != a_out.end() ? xptr[ia - a_out.begin()]
+                                                     : yptr[std::find(b_out.begin(), b_out.end(), i) - b_out.begin()];
+            const int w = width_of(src->type);
+            EF_TRY(c.alloc(kd[i], total * w));
+            EF_TRY(c.alloc(kv[i], total));
+            keys.src[i] = src->raw;
+            keys.src_valid[i] = src->valid;
+            keys.dst[i] = kd[i];
+            keys.dst_valid[i] = kv[i];
+            keys.width[i] = w;
+            keys.from_a[i] = ia != a_out.end();
+        }
+        double *vals;
+        uint8_t *valid;
+        EF_TRY(c.alloc(vals, total));
+        EF_TRY(c.alloc(valid, total));
+        EF_TRY(emit_groups(nx, bound_off, out_off, g_count, g_y, g_sum, g_valued, x_first, y_first, keys, vals, valid,
+                           c.stream));
+        std::vector<OutColumn> cols;
+        for (size_t i = 0; i < m.outputs.size(); i++) {
+            OutColumn k;
+            EF_TRY(fetch_column(c, fields[i].type, kd[i], kv[i], (int64_t)total, fields[i].nullable, k));
+            cols.push_back(std::move(k));
+        }
+        OutColumn v;
+        EF_TRY(fetch_column(c, Type::Float64, vals, valid, (int64_t)total, true, v));
+        cols.push_back(std::move(v));
+        result.push_back(std::move(cols));
+        return EF_OK;
+    }
+    if (!domain) {
+        set_error("a fold with a domain was given no domain relation");
+        return EF_INVALID;
+    }
+    // Domain rows find their group by rank (discriminate.rs:611-648).
+    uint8_t *matches;
+    EF_TRY(c.alloc(matches, (size_t)nd));
+    if (nd) EF_HIP(hipMemsetAsync(matches, 1, (size_t)nd, c.stream));
+    for (size_t j = 0; j < a_out.size(); j++)
+        if (m.outputs[a_out[j]].domain_equality == EF_EQUAL) EF_TRY(and_valid(matches, xptr[j]->valid + na, nd, c.stream));
+    for (size_t j = 0; j < b_out.size(); j++)
+        if (m.outputs[b_out[j]].domain_equality == EF_EQUAL) EF_TRY(and_valid(matches, yptr[j]->valid + nb, nd, c.stream));
+    int64_t off = 0;
+    for (size_t bi = 0; bi < domain->batches.size(); bi++) {
+        const int64_t n = domain->batch_rows[bi];
+        double *vals;
+        uint8_t *valid;
+        EF_TRY(c.alloc(vals, (size_t)n));
+        EF_TRY(c.alloc(valid, (size_t)n));
+        EF_TRY(domain_lookup(n, matches + off, x_rank + na + off, y_rank + nb + off, nx, bound_off, g_count, g_y, g_sum,
+                             g_valued, m.has_missing, m.missing, vals, valid, c.stream));
+        std::vector<OutColumn> cols;
+        for (size_t i = 0; i < m.carry.size(); i++)
+            cols.push_back(copy_column(domain->batches[bi][m.carry[i]], n, fields[i].nullable));
+        OutColumn v;
+        EF_TRY(fetch_column(c, Type::Float64, vals, m.has_missing ? nullptr : valid, n, fields.back().nullable, v));
+        cols.push_back(std::move(v));
+        result.push_back(std::move(cols));
+        off += n;
+    }
+    return EF_OK;
+}
+
+}  // namespace ef
+
+using namespace ef;
+
+namespace {
+
+// Reads the relations and computes the schema; EF_OK to go on.
+ef_status prepare(const FoldModel &m, const ef_relation *a, const ef_relation *b, const ef_relation *domain,
+                  const ef_options *options, Relation &ra, Relation &rb, Relation &rd, bool &has_domain,
+                  std::vector<OutField> &fields) {
+    if (!read_relation(a, ra) || !read_relation(b, rb)) return EF_INVALID;
+    has_domain = domain != nullptr;
+    if (has_domain && !read_relation(domain, rd)) return EF_INVALID;
+    std::vector<const Relation *> ops = {&ra, &rb};
+    if (m.operands.size() != 2) return EF_DECLINED;
+    if (!fold_schema(m, ops, has_domain ? &rd : nullptr, options && options->accept_float32, fields))
+        return EF_INVALID;
+    return EF_OK;
+}
+
+// Runs `body` catching host allocation failures, so no exception crosses
+// the ABI.
+template <class F> ef_status guarded(F body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        set_error("host memory exhausted");
+        return EF_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return EF_DEVICE_ERROR;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void ef_result_release(ef_result *r) {
+    if (!r) return;
+    for (size_t i = 0; i < r->n_batches; i++)
+        if (r->batches[i].release) r->batches[i].release(&r->batches[i]);
+    delete[] r->batches;
+    r->batches = nullptr;
+    r->n_batches = 0;
+    if (r->schema.release) r->schema.release(&r->schema);
+}
+
+ef_status ef_fold_dense(const ef_fold_desc *fold, const ef_relation *a, const ef_relation *b,
+                        const ef_relation *domain, const ef_options *options, ef_result *out) {
+    clear_error();
+    if (!out) {
+        set_error("no result to write");
+        return EF_INVALID;
+    }
+    std::memset(out, 0, sizeof *out);
+    return guarded([&]() -> ef_status {
+        FoldModel m;
+        if (!parse_fold(fold, m)) return EF_INVALID;
+        Relation ra, rb, rd;
+        bool has_domain;
+        std::vector<OutField> fields;
+        EF_TRY(prepare(m, a, b, domain, options, ra, rb, rd, has_domain, fields));
+        Call c(options);
+        std::vector<std::vector<OutColumn>> batches;
+        ef_status st = fold_dense_impl(m, ra, rb, has_domain ? &rd : nullptr, fields, c, batches);
+        if (st != EF_OK) {
+            (void)hipStreamSynchronize(c.stream);
+            return st;
+        }
+        put_result(std::move(batches), fields, EF_ALGORITHM_DENSE, out);
+        return EF_OK;
+    });
+}
+
+ef_status ef_fold(const ef_fold_desc *fold, const ef_relation *a, const ef_relation *b, const ef_relation *domain,
+                  const ef_options *options, ef_result *out) {
+    clear_error();
+    if (!out) {
+        set_error("no result to write");
+        return EF_INVALID;
+    }
+    std::memset(out, 0, sizeof *out);
+    return guarded([&]() -> ef_status {
+        FoldModel m;
+        if (!parse_fold(fold, m)) return EF_INVALID;
+        if (m.operands.size() != 2) {
+            set_error("a fold of two operands");
+            return EF_INVALID;
+        }
+        if (options && options->domain_rows == EF_DOMAIN_ROWS_UNIQUE_KEYS) {
+            set_error("only fold_dense checks that a domain's keys are unique");
+            return EF_INVALID;
+        }
+        Relation ra, rb, rd;
+        bool has_domain;
+        std::vector<OutField> fields;
+        EF_TRY(prepare(m, a, b, domain, options, ra, rb, rd, has_domain, fields));
+        std::vector<std::vector<OutColumn>> batches;
+        {
+            Call c(options);
+            ef_status st = fold_dense_impl(m, ra, rb, has_domain ? &rd : nullptr, fields, c, batches);
+            (void)hipStreamSynchronize(c.stream);
+            if (st == EF_OK) {
+                put_result(std::move(batches), fields, EF_ALGORITHM_DENSE, out);
+                return EF_OK;
+            }
+            if (st != EF_DECLINED) return st;
+        }
+        // algorithm.rs:114-133: the sorting algorithm computes no masks, and
+        // takes each operand as one batch.
+        if (!m.conditions.empty()) {
+            set_error("the discrimination algorithm computes no masks");
+            return EF_INVALID;
+        }
+        Call c(options);
+        EF_TRY(c.reserve((double)relation_bytes(ra) + (double)relation_bytes(rb), "the operands as one batch each"));
+        batches.clear();
+        ef_status st = fold_discriminate_impl(m, ra, rb, has_domain ? &rd : nullptr, fields, c, batches);
+        if (st != EF_OK) {
+            (void)hipStreamSynchronize(c.stream);
+            return st;
+        }
+        put_result(std::move(batches), fields, EF_ALGORITHM_DISCRIMINATE, out);
+        return EF_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Host-side pieces shared by the operator entry points: the fold
+// description read into plain vectors, the output schema, the budget and
+// device allocations of one call, and uploads of Arrow columns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <deque>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "arrow_io.h"
+#include "common.h"
+
+namespace ef {
+
+// One operand of a fold (fold.rs:101-111).
+struct OperandModel {
+    std::vector<std::pair<uint32_t, uint32_t>> keys;   // (dimension, column)
+    int factor = -1;                                   // -1: a mask
+    int column_of(uint32_t d) const {
+        for (auto &k : keys)
+            if (k.first == d) return (int)k.second;
+        return -1;
+    }
+};
+
+// A fold (fold.rs:380-388) checked for indices in range.
+struct FoldModel {
+    std::vector<uint32_t> equality;                   // per dimension
+    std::vector<OperandModel> operands;
+    std::vector<ef_output> outputs;
+    std::vector<ef_condition> conditions;
+    bool has_domain = false;
+    std::vector<uint32_t> carry;
+    bool has_missing = false;
+    double missing = 0.0;
+    std::vector<ef_expr_op> row;                       // empty: the product
+    int output_of(uint32_t d) const {
+        for (size_t i = 0; i < outputs.size(); i++)
+            if (outputs[i].dimension == d) return (int)i;
+        return -1;
+    }
+    bool aggregated(uint32_t d) const { return output_of(d) < 0; }
+};
+
+bool parse_fold(const ef_fold_desc *f, FoldModel &m);
+
+// schema.rs:40-143: output keys typed as the first operand holding each
+// dimension, then a Float64 value, nullable unless a domain gives a
+// default.  With a domain, its carried columns come first instead.
+bool fold_schema(const FoldModel &m, const std::vector<const Relation *> &operands, const Relation *domain,
+                 bool accept_float32, std::vector<OutField> &fields);
+
+// One call's options, stream, budget and device memory.
+struct Call {
+    const ef_options *options = nullptr;
+    hipStream_t stream = nullptr;
+    std::deque<DeviceBuffer> buffers;
+
+    explicit Call(const ef_options *o) : options(o), stream(as_stream(o ? o->stream : nullptr)) {}
+
+    // options.rs:95-103: asks the budget, EF_OUT_OF_MEMORY if it says no.
+    ef_status reserve(double bytes, const char *what);
+
+    template <class T> ef_status alloc(T *&p, size_t n) {
+        buffers.emplace_back();
+        ef_status st = buffers.back().alloc(n * sizeof(T) + 16);
+        p = buffers.back().as<T>();
+        return st;
+    }
+    template <class T> ef_status zeros(T *&p, size_t n) {
+        EF_TRY(alloc(p, n));
+        EF_HIP(hipMemsetAsync(p, 0, n * sizeof(T) + 16, stream));
+        return EF_OK;
+    }
+    template <class T> ef_status upload(T *dst, const void *src, size_t n) {
+        if (n) EF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream));
+        return EF_OK;
+    }
+    template <class T> ef_status download(void *dst, const T *src, size_t n) {
+        if (n) EF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+        return EF_OK;
+    }
+    ef_status sync() {
+        EF_HIP(hipStreamSynchronize(stream));
+        return EF_OK;
+    }
+};
+
+// A column of every batch of a relation, on the device as one array:
+// factors as f64 with a validity byte per row, keys as i64.
+ef_status upload_factor(Call &c, const Relation &r, int column, double *values, uint8_t *valid, bool *any_null);
+ef_status upload_validity(Call &c, const Relation &r, int column, uint8_t *valid);
+ef_status upload_raw(Call &c, const Relation &r, int column, void *dst);
+
+// Host copy of one input column into an owned output column (a domain's
+// carried columns).
+OutColumn copy_column(const ColumnView &v, int64_t rows, bool nullable);
+
+// Builds an output column from device values (width bytes each) and an
+// optional device byte-per-row validity; synchronizes.
+ef_status fetch_column(Call &c, Type t, const void *values, const uint8_t *valid, int64_t n, bool nullable,
+                       OutColumn &out);
+
+// Moves batches and a schema into an ef_result.
+void put_result(std::vector<std::vector<OutColumn>> batches, const std::vector<OutField> &fields, uint32_t algorithm,
+                ef_result *out);
+
+}  // namespace ef

@@ -84,6 +84,8 @@ ef_status emit(int64_t n, uint64_t ny, const uint32_t *rows, const uint32_t *kee
 ef_status domain_values(int64_t n, const int64_t *cell, uint64_t ny, const double *out, const uint8_t *valued,
                         const Conds &c, int has_missing, double missing, double *values, uint8_t *valid,
                         hipStream_t s);
+ef_status unpack_bits(int64_t n, const uint8_t *bits, int64_t offset, uint8_t *out, hipStream_t s);
+ef_status widen_keys(Type t, int64_t n, const void *in, uint8_t *valid, int64_t *out, hipStream_t s);
 ef_status pack_bits(int64_t n, const uint8_t *bytes, uint8_t *bits, unsigned long long *nulls, hipStream_t s);
 
 }  // namespace ef

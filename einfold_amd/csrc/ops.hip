@@ -83,7 +83,8 @@ __global__ __launch_bounds__(256) void place_a_kernel(int64_t n, Dims xs, Dims z
     }
 }
 
-// b's rows placed into [w][z][y] (dense.rs:372-386), holes left 0.
+// b's rows placed into [w][z][y] (dense.rs:372-386), holes left 0; only
+// a's factors need be finite (dense.rs:30-35).
 __global__ __launch_bounds__(256) void place_b_kernel(int64_t n, Dims wzs, Dims ys, uint64_t ny,
                                                       const double *__restrict__ f, const uint8_t *__restrict__ fvalid,
                                                       uint32_t *present, double *b, PlaceFlags *flags) {
@@ -94,7 +95,6 @@ __global__ __launch_bounds__(256) void place_b_kernel(int64_t n, Dims wzs, Dims 
     if (fvalid[r]) {
         const double v = f[r];
         b[p] = v;
-        if (!isfinite(v)) atomicOr(&flags->not_finite, 1);
     } else {
         atomicOr(&flags->nulls, 1);
     }
@@ -230,6 +230,29 @@ __global__ __launch_bounds__(256) void u8_to_u32_kernel(int64_t n, const uint8_t
     if (i < n) out[i] = in[i];
 }
 
+// An Arrow validity bitmap (bit `offset` onward) as one byte per row.
+__global__ __launch_bounds__(256) void unpack_bits_kernel(int64_t n, const uint8_t *__restrict__ bits, int64_t offset,
+                                                          uint8_t *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t b = offset + r;
+    out[r] = (bits[b >> 3] >> (b & 7)) & 1;
+}
+
+// A domain's key column as i64 with validity: NULLs and unsigned keys of
+// 2^63 or more locate no cell (dense.rs:595-615).
+template <class T>
+__global__ __launch_bounds__(256) void widen_keys_kernel(int64_t n, const T *__restrict__ in, uint8_t *valid,
+                                                         int64_t *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const T v = in[r];
+    if constexpr (sizeof(T) == 8 && !std::is_signed<T>::value) {
+        if ((uint64_t)v >> 63) { valid[r] = 0; out[r] = 0; return; }
+    }
+    out[r] = (int64_t)v;
+}
+
 }  // namespace
 
 ef_status cast_keys(Type t, const void *in, int64_t n, int64_t *out, KeyStats *stats, hipStream_t s) {
@@ -339,6 +362,33 @@ ef_status domain_values(int64_t n, const int64_t *cell, uint64_t ny, const doubl
 ef_status pack_bits(int64_t n, const uint8_t *bytes, uint8_t *bits, unsigned long long *nulls, hipStream_t s) {
     if (n == 0) return EF_OK;
     hipLaunchKernelGGL(pack_bits_kernel, dim3(blocks_for((n + 7) / 8, 256)), dim3(256), 0, s, n, bytes, bits, nulls);
+    EF_HIP(hipGetLastError());
+    return EF_OK;
+}
+
+ef_status unpack_bits(int64_t n, const uint8_t *bits, int64_t offset, uint8_t *out, hipStream_t s) {
+    if (n == 0) return EF_OK;
+    hipLaunchKernelGGL(unpack_bits_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, bits, offset, out);
+    EF_HIP(hipGetLastError());
+    return EF_OK;
+}
+
+ef_status widen_keys(Type t, int64_t n, const void *in, uint8_t *valid, int64_t *out, hipStream_t s) {
+    if (n == 0) return EF_OK;
+    const unsigned g = blocks_for(n, 256);
+#define EF_W(T) hipLaunchKernelGGL(widen_keys_kernel<T>, dim3(g), dim3(256), 0, s, n, (const T *)in, valid, out)
+    switch (t) {
+    case Type::Int8: EF_W(int8_t); break;
+    case Type::UInt8: EF_W(uint8_t); break;
+    case Type::Int16: EF_W(int16_t); break;
+    case Type::UInt16: EF_W(uint16_t); break;
+    case Type::Int32: EF_W(int32_t); break;
+    case Type::UInt32: EF_W(uint32_t); break;
+    case Type::Int64: EF_W(int64_t); break;
+    case Type::UInt64: EF_W(uint64_t); break;
+    default: set_error("a domain's keys must be integers"); return EF_INVALID;
+    }
+#undef EF_W
     EF_HIP(hipGetLastError());
     return EF_OK;
 }

@@ -194,6 +194,99 @@ size_t rank_scratch_bytes(size_t n) {
            scan_temp_bytes(counts > n ? counts : n) + 1024;
 }
 
+namespace {
+
+// Scratch layout shared by ranking and sorting.
+struct RadixScratch {
+    uint64_t *wa, *wb;
+    uint32_t *ra, *rb, *flags, *counts, *offs;
+    unsigned long long *bits;
+    void *tmp;
+    size_t counts_n;
+    int64_t tiles;
+};
+
+RadixScratch carve(void *scratch, size_t n) {
+    RadixScratch r;
+    r.tiles = (int64_t)((n + TILE - 1) / TILE);
+    r.counts_n = 256 * (size_t)r.tiles;
+    char *p = static_cast<char *>(scratch);
+    r.wa = reinterpret_cast<uint64_t *>(p); p += round_up(n * 8);
+    r.wb = reinterpret_cast<uint64_t *>(p); p += round_up(n * 8);
+    r.ra = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
+    r.rb = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
+    r.flags = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
+    r.counts = reinterpret_cast<uint32_t *>(p); p += round_up(r.counts_n * 4);
+    r.offs = reinterpret_cast<uint32_t *>(p); p += round_up(r.counts_n * 4);
+    r.bits = reinterpret_cast<unsigned long long *>(p); p += 256;
+    r.tmp = p;
+    return r;
+}
+
+// The stable LSD passes over the bytes some word varies in
+// (discriminate.rs:113-156).  On return (w_cur, r_cur) hold the sorted
+// pairs; r_cur is null when no byte varies (the input order stands).
+ef_status radix_passes(const uint64_t *words, const uint32_t *rows_in, size_t n, const RadixScratch &x,
+                       const uint64_t *&w_cur, const uint32_t *&r_cur, hipStream_t s) {
+    const unsigned long long init[2] = {0ULL, ~0ULL};
+    EF_HIP(hipMemcpyAsync(x.bits, init, sizeof init, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(or_and_kernel, dim3(256), dim3(256), 0, s, words, (int64_t)n, x.bits, x.bits + 1);
+    unsigned long long host_bits[2];
+    EF_HIP(hipMemcpyAsync(host_bits, x.bits, sizeof host_bits, hipMemcpyDeviceToHost, s));
+    EF_HIP(hipStreamSynchronize(s));
+    const uint64_t varies = host_bits[0] ^ host_bits[1];
+    w_cur = words;
+    r_cur = rows_in;
+    bool sorted_any = false, in_a = false;
+    for (int shift = 0; shift < 64; shift += 8) {
+        if (((varies >> shift) & 0xff) == 0) continue;
+        hipLaunchKernelGGL(histogram_kernel, dim3((unsigned)x.tiles), dim3(256), 0, s, w_cur, (int64_t)n, shift,
+                           x.counts, x.tiles);
+        EF_TRY(scan_u32(x.counts, x.offs, x.counts_n, x.tmp, s));
+        uint64_t *w_next = in_a ? x.wb : x.wa;
+        uint32_t *r_next = in_a ? x.rb : x.ra;
+        hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)x.tiles), dim3(256), 0, s, w_cur, r_cur, w_next, r_next,
+                           (int64_t)n, shift, x.offs, x.tiles);
+        w_cur = w_next;
+        r_cur = r_next;
+        in_a = !in_a;
+        sorted_any = true;
+    }
+    if (!sorted_any) r_cur = nullptr;
+    EF_HIP(hipGetLastError());
+    return EF_OK;
+}
+
+__global__ __launch_bounds__(256) void iota_kernel(uint32_t *out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)i;
+}
+
+}  // namespace
+
+ef_status sort_u64(const uint64_t *words, const uint32_t *rows_in, size_t n, uint64_t *w_out, uint32_t *r_out,
+                   void *scratch, hipStream_t s) {
+    if (n == 0) return EF_OK;
+    if (n > 0xffffffffULL) {
+        set_error("sorting takes at most 2^32 - 1 rows");
+        return EF_INVALID;
+    }
+    const RadixScratch x = carve(scratch, n);
+    const uint64_t *w_cur;
+    const uint32_t *r_cur;
+    EF_TRY(radix_passes(words, rows_in, n, x, w_cur, r_cur, s));
+    if (w_out) EF_HIP(hipMemcpyAsync(w_out, w_cur, n * 8, hipMemcpyDeviceToDevice, s));
+    if (r_cur) {
+        EF_HIP(hipMemcpyAsync(r_out, r_cur, n * 4, hipMemcpyDeviceToDevice, s));
+    } else if (rows_in) {
+        EF_HIP(hipMemcpyAsync(r_out, rows_in, n * 4, hipMemcpyDeviceToDevice, s));
+    } else {
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, r_out, (int64_t)n);
+    }
+    EF_HIP(hipGetLastError());
+    return EF_OK;
+}
+
 ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *first,
                    size_t *n_ranks, void *scratch, hipStream_t s) {
     if (n_ranks) *n_ranks = 0;
@@ -207,44 +300,10 @@ ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *fi
         EF_TRY(own.alloc(rank_scratch_bytes(n)));
         scratch = own.ptr;
     }
-    const int64_t tiles = (int64_t)((n + TILE - 1) / TILE);
-    const size_t counts_n = 256 * (size_t)tiles;
-    char *p = static_cast<char *>(scratch);
-    uint64_t *wa = reinterpret_cast<uint64_t *>(p); p += round_up(n * 8);
-    uint64_t *wb = reinterpret_cast<uint64_t *>(p); p += round_up(n * 8);
-    uint32_t *ra = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
-    uint32_t *rb = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
-    uint32_t *flags = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
-    uint32_t *counts = reinterpret_cast<uint32_t *>(p); p += round_up(counts_n * 4);
-    uint32_t *offs = reinterpret_cast<uint32_t *>(p); p += round_up(counts_n * 4);
-    unsigned long long *bits = reinterpret_cast<unsigned long long *>(p); p += 256;
-    void *tmp = p;
-
-    // Which bytes vary (discriminate.rs:113-116).
-    const unsigned long long init[2] = {0ULL, ~0ULL};
-    EF_HIP(hipMemcpyAsync(bits, init, sizeof init, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(or_and_kernel, dim3(256), dim3(256), 0, s, words, (int64_t)n, bits, bits + 1);
-    unsigned long long host_bits[2];
-    EF_HIP(hipMemcpyAsync(host_bits, bits, sizeof host_bits, hipMemcpyDeviceToHost, s));
-    EF_HIP(hipStreamSynchronize(s));
-    const uint64_t varies = host_bits[0] ^ host_bits[1];
-
-    const uint64_t *w_cur = words;
-    const uint32_t *r_cur = nullptr;
-    bool in_a = false;
-    for (int shift = 0; shift < 64; shift += 8) {
-        if (((varies >> shift) & 0xff) == 0) continue;
-        hipLaunchKernelGGL(histogram_kernel, dim3((unsigned)tiles), dim3(256), 0, s, w_cur, (int64_t)n, shift,
-                           counts, tiles);
-        EF_TRY(scan_u32(counts, offs, counts_n, tmp, s));
-        uint64_t *w_next = in_a ? wb : wa;
-        uint32_t *r_next = in_a ? rb : ra;
-        hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)tiles), dim3(256), 0, s, w_cur, r_cur, w_next, r_next,
-                           (int64_t)n, shift, offs, tiles);
-        w_cur = w_next;
-        r_cur = r_next;
-        in_a = !in_a;
-    }
+    const RadixScratch x = carve(scratch, n);
+    const uint64_t *w_cur;
+    const uint32_t *r_cur;
+    EF_TRY(radix_passes(words, nullptr, n, x, w_cur, r_cur, s));
     if (!r_cur) {
         // Every word is the same: one rank, first row 0, and the identity
         // order.
@@ -255,14 +314,14 @@ ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *fi
         if (n_ranks) *n_ranks = 1;
         return EF_OK;
     }
-    hipLaunchKernelGGL(flag_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w_cur, (int64_t)n, flags);
+    hipLaunchKernelGGL(flag_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w_cur, (int64_t)n, x.flags);
     // The row buffer the last pass did not write is free for the scan.
-    uint32_t *excl = r_cur == ra ? rb : ra;
-    EF_TRY(scan_u32(flags, excl, n, tmp, s));
+    uint32_t *excl = r_cur == x.ra ? x.rb : x.ra;
+    EF_TRY(scan_u32(x.flags, excl, n, x.tmp, s));
     hipLaunchKernelGGL(assign_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w_cur, r_cur, excl, (int64_t)n,
                        rank, first);
-    uint32_t *total = reinterpret_cast<uint32_t *>(bits);
-    hipLaunchKernelGGL(count_kernel, dim3(1), dim3(1), 0, s, excl, flags, (int64_t)n, total);
+    uint32_t *total = reinterpret_cast<uint32_t *>(x.bits);
+    hipLaunchKernelGGL(count_kernel, dim3(1), dim3(1), 0, s, excl, x.flags, (int64_t)n, total);
     uint32_t host_total = 0;
     EF_HIP(hipMemcpyAsync(&host_total, total, 4, hipMemcpyDeviceToHost, s));
     EF_HIP(hipStreamSynchronize(s));
