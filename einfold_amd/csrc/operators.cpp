@@ -5,11 +5,13 @@
 #include "operators.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <limits>
 
 #include "discriminate.h"
+#include "pipeline.h"
 #include "ops.h"
 
 namespace ef {
@@ -1045,6 +1047,496 @@ ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Re
     return EF_OK;
 }
 
+// ---------------------------------------------------------------------
+// The fused pipeline (fused.rs:151-489)
+// ---------------------------------------------------------------------
+
+namespace {
+
+// Each operand's keys as i64 per dimension (every batch in one array),
+// with the dimensions' extents; false if dense keys don't hold.
+struct KeyedOperand {
+    const Relation *rel = nullptr;
+    const OperandModel *op = nullptr;
+    std::vector<int64_t *> key;        // by dimension
+};
+
+ef_status cast_operand_keys(Call &c, KeyedOperand &ko, size_t n_dims, KeyStats *stats) {
+    ko.key.assign(n_dims, nullptr);
+    const Relation &r = *ko.rel;
+    size_t raw_bytes = 0;
+    for (auto &k : ko.op->keys) {
+        if (!r.batches.empty() && !is_integer(r.types[k.second])) return EF_DECLINED;
+        for (auto &bt : r.batches)
+            if (bt[k.second].null_count) return EF_DECLINED;
+        raw_bytes = std::max(raw_bytes, (size_t)r.rows * width_of(r.types[k.second]));
+    }
+    uint8_t *raw;
+    EF_TRY(c.alloc(raw, raw_bytes));
+    for (auto &k : ko.op->keys) {
+        EF_TRY(c.alloc(ko.key[k.first], (size_t)r.rows));
+        if (!r.rows) continue;
+        EF_TRY(upload_raw(c, r, (int)k.second, raw));
+        EF_TRY(cast_keys(r.types[k.second], raw, r.rows, ko.key[k.first], stats + k.first, c.stream));
+    }
+    return EF_OK;
+}
+
+// Whether a postfix program is well formed for the device evaluator: every
+// operator finds its operands, the stack never holds more than 16 values,
+// and one value is left.
+bool program_ok(const ef_expr_op *ops, size_t n) {
+    int depth = 0;
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t code = ops[i].code;
+        if (code == EF_EXPR_COLUMN || code == EF_EXPR_LITERAL) depth++;
+        else if (code >= EF_EXPR_NEG && code <= EF_EXPR_LN) { if (depth < 1) return false; }
+        else if (code >= EF_EXPR_ADD && code <= EF_EXPR_POW) { if (depth < 2) return false; depth--; }
+        else return false;
+        if (depth > 16) return false;
+    }
+    return depth == 1;
+}
+
+struct Placement {
+    int operand;          // 0 q, 1 k, 2 v
+    int column;
+    int slot;             // index among the operand's placed columns
+};
+
+}  // namespace
+
+ef_status pipeline_impl(const ef_pipeline_desc *pd, const FoldModel &prod, const Relation &q, const Relation &k,
+                        const Relation &v, const std::vector<OutField> &fields, Call &c,
+                        std::vector<std::vector<OutColumn>> &result, uint64_t &tiles_visited, uint64_t &tiles) {
+    const bool accept32 = c.options && c.options->accept_float32;
+    if (prod.operands.size() != 2) return EF_DECLINED;
+    const OperandModel &oq = prod.operands[0], &ok_ = prod.operands[1];
+    OperandModel ov;
+    ov.factor = pd->values.factor;
+    for (uint32_t i = 0; i < pd->values.n_keys; i++)
+        ov.keys.emplace_back(pd->values.keys[i].dimension, pd->values.keys[i].column);
+    const uint32_t own = (uint32_t)prod.equality.size();
+    const uint32_t n = own + pd->n_dimensions;
+    std::vector<uint32_t> outs(pd->outputs, pd->outputs + pd->n_outputs);
+    auto output = [&](uint32_t d) { return std::find(outs.begin(), outs.end(), d) != outs.end(); };
+    auto holds = [](const OperandModel &o, uint32_t d) { return o.column_of(d) >= 0; };
+    // Roles (fused.rs:171-189): W batches, X queries, E features, J keys.
+    std::vector<uint32_t> ws, xs, es, js, ys;
+    for (uint32_t d = 0; d < own; d++) {
+        const bool produced = prod.output_of(d) >= 0;
+        const int key = (holds(oq, d) << 4) | (holds(ok_, d) << 3) | (produced << 2) | (output(d) << 1) | holds(ov, d);
+        if (key == 0b11111) ws.push_back(d);
+        else if (key == 0b10110) xs.push_back(d);
+        else if (key == 0b11000) es.push_back(d);
+        else if (key == 0b01101) js.push_back(d);
+        else return EF_DECLINED;
+    }
+    for (uint32_t d = own; d < n; d++) {
+        if (!output(d)) return EF_DECLINED;
+        ys.push_back(d);
+    }
+    // Masks: conditions between the one X and the one J dimension
+    // (fused.rs:190-215), mirrored to read x <cmp> j + offset.
+    std::vector<ef_condition> mask;
+    for (const ef_condition &cd : prod.conditions) {
+        if (xs.size() == 1 && js.size() == 1 && cd.left == xs[0] && cd.right == js[0]) {
+            mask.push_back(cd);
+        } else if (xs.size() == 1 && js.size() == 1 && cd.left == js[0] && cd.right == xs[0]) {
+            if (cd.offset == std::numeric_limits<int64_t>::min()) return EF_DECLINED;
+            ef_condition mc = cd;
+            mc.left = cd.right;
+            mc.right = cd.left;
+            mc.offset = -cd.offset;
+            static const uint32_t mirror[4] = {EF_GREATER, EF_GREATER_OR_EQUAL, EF_LESS, EF_LESS_OR_EQUAL};
+            mc.comparison = mirror[cd.comparison];
+            mask.push_back(mc);
+        } else {
+            return EF_DECLINED;
+        }
+    }
+    const std::vector<uint32_t> local_xs = xs, local_js = js;
+    xs.insert(xs.begin(), ws.begin(), ws.end());
+    js.insert(js.begin(), ws.begin(), ws.end());
+    // What the producer reads (fused.rs:229-262).
+    std::vector<Placement> reads;
+    std::vector<ef_expr_op> score(prod.row);
+    int nq_cols = 0, nk_cols = 0;
+    if (prod.row.empty()) {
+        if (oq.factor < 0 || ok_.factor < 0) return EF_DECLINED;
+        reads.push_back({0, oq.factor, 0});
+        reads.push_back({1, ok_.factor, 0});
+    } else {
+        for (auto &e : score) {
+            if (e.code != EF_EXPR_COLUMN) continue;
+            int slot = -1;
+            for (auto &r : reads)
+                if (r.operand == (int)e.operand && r.column == (int)e.column) slot = r.slot;
+            if (slot < 0) {
+                if (e.operand > 1) return EF_DECLINED;
+                slot = e.operand == 0 ? nq_cols++ : nk_cols++;
+                reads.push_back({(int)e.operand, (int)e.column, slot});
+            }
+            e.column = (uint32_t)slot;
+        }
+        if (nq_cols == 0 || nk_cols == 0 || nq_cols > 8 || nk_cols > 8) return EF_DECLINED;
+    }
+    if (ov.factor < 0) return EF_DECLINED;
+    if ((!score.empty() && !program_ok(score.data(), score.size())) ||
+        (pd->weight_kind == 1 && !program_ok(pd->weight, pd->n_weight))) {
+        set_error("a row expression the device can't evaluate");
+        return EF_DECLINED;
+    }
+    if (xs.size() > MAX_DIMS || js.size() > MAX_DIMS || es.size() > MAX_DIMS || ys.size() > MAX_DIMS ||
+        outs.size() > MAX_DIMS)
+        return EF_DECLINED;
+
+    // Keys and extents (fused.rs:263-300).
+    KeyStats *stats;
+    EF_TRY(c.alloc(stats, n ? n : 1));
+    {
+        std::vector<KeyStats> init(n ? n : 1, KeyStats{0, -1, 0, 0});
+        EF_TRY(c.upload<KeyStats>(stats, init.data(), init.size()));
+    }
+    KeyedOperand kq{&q, &oq, {}}, kk{&k, &ok_, {}}, kv{&v, &ov, {}};
+    for (KeyedOperand *ko : {&kq, &kk, &kv}) {
+        const ef_status st = cast_operand_keys(c, *ko, n, stats);
+        if (st != EF_OK) return st;
+    }
+    std::vector<KeyStats> st(n ? n : 1);
+    EF_TRY(c.download<KeyStats>(st.data(), stats, st.size()));
+    EF_TRY(c.sync());
+    std::vector<uint64_t> extent(n, 0);
+    for (uint32_t d = 0; d < n; d++) {
+        if (st[d].bad || st[d].lo < 0 || st[d].hi == std::numeric_limits<int64_t>::max()) return EF_DECLINED;
+        extent[d] = (uint64_t)(st[d].hi + 1);
+    }
+    const uint64_t nx = size_of(xs, extent), ne = size_of(es, extent), nj = size_of(js, extent),
+                   ny = size_of(ys, extent);
+    const uint64_t local_nx = size_of(local_xs, extent), local_nj = size_of(local_js, extent);
+    auto dims = [&](const std::vector<uint32_t> &role, const std::vector<int64_t *> &keys) {
+        Dims d{};
+        d.n = (int)role.size();
+        for (size_t i = 0; i < role.size(); i++) {
+            d.key[i] = keys[role[i]];
+            d.extent[i] = (int64_t)extent[role[i]];
+        }
+        return d;
+    };
+    // Placement (fused.rs:624-664): exactly one non-NULL row per cell.
+    uint8_t *ones;
+    const int64_t max_rows = std::max({q.rows, k.rows, v.rows, (int64_t)1});
+    EF_TRY(c.alloc(ones, (size_t)max_rows));
+    EF_HIP(hipMemsetAsync(ones, 1, (size_t)max_rows, c.stream));
+    PlaceFlags *flags;
+    EF_TRY(c.zeros(flags, 1));
+    bool any32 = false;
+    auto place = [&](const Relation &r, const KeyedOperand &ko, const std::vector<uint32_t> &rd,
+                     const std::vector<uint32_t> &cd, int column, double *&m) -> ef_status {
+        const uint64_t rows = size_of(rd, extent), cols = size_of(cd, extent);
+        const uint64_t cells = mul_sat(rows, cols);
+        if ((uint64_t)r.rows != cells || cells >= U32_MAX) return EF_DECLINED;
+        if (column < 0 || (size_t)column >= r.columns()) return EF_DECLINED;
+        if (!r.batches.empty() && !factor_type_ok(r.types[column], accept32)) return EF_DECLINED;
+        for (auto &bt : r.batches)
+            if (bt[column].null_count) return EF_DECLINED;
+        if (!r.batches.empty() && r.types[column] == Type::Float32) any32 = true;
+        double *f;
+        uint32_t *present;
+        EF_TRY(c.alloc(m, cells));
+        EF_TRY(c.alloc(f, cells));
+        EF_TRY(c.zeros(present, cells));
+        if (cells) EF_TRY(upload_factor(c, r, column, f, nullptr, nullptr));
+        return place_b((int64_t)cells, dims(rd, ko.key), dims(cd, ko.key), cols, f, ones, present, m, flags,
+                       c.stream);
+    };
+    double *qm = nullptr, *km = nullptr, *vm = nullptr;
+    std::vector<double *> qcols(8, nullptr), kcols(8, nullptr);
+    for (const Placement &pl : reads) {
+        double *m;
+        const ef_status s2 = pl.operand == 0 ? place(q, kq, xs, es, pl.column, m) : place(k, kk, es, js, pl.column, m);
+        if (s2 != EF_OK) return s2;
+        (pl.operand == 0 ? qcols : kcols)[pl.slot] = m;
+    }
+    qm = qcols[0];
+    km = kcols[0];
+    {
+        const ef_status s2 = place(v, kv, js, ys, ov.factor, vm);
+        if (s2 != EF_OK) return s2;
+    }
+    PlaceFlags pf;
+    EF_TRY(c.download<PlaceFlags>(&pf, flags, 1));
+    EF_TRY(c.sync());
+    if (pf.duplicate) return EF_DECLINED;
+    if (mul_sat(nx, ny) >= U32_MAX) return EF_DECLINED;
+
+    // Each x's allowed keys (fused.rs:350-368), global j.
+    std::vector<int64_t> lo(nx), hi(nx);
+    const uint64_t lnx = local_nx ? local_nx : 1;
+    for (uint64_t x = 0; x < nx; x++) {
+        const __int128 xl = (__int128)(x % lnx);
+        __int128 l = 0, h = (__int128)local_nj;
+        for (const ef_condition &cd : mask) {
+            const __int128 bound = xl - (__int128)cd.offset;
+            switch (cd.comparison) {
+            case EF_GREATER_OR_EQUAL: h = std::min(h, bound + 1); break;
+            case EF_GREATER: h = std::min(h, bound); break;
+            case EF_LESS_OR_EQUAL: l = std::max(l, bound); break;
+            default: l = std::max(l, bound + 1); break;
+            }
+        }
+        h = std::max(h, l);
+        auto clamp = [&](__int128 t) { return (int64_t)std::min<__int128>(std::max<__int128>(t, 0), local_nj); };
+        const int64_t gs = (int64_t)((x / lnx) * local_nj);
+        lo[x] = clamp(l) + gs;
+        hi[x] = clamp(h) + gs;
+    }
+    // Tile pairs (fused.rs:385-397) and those visited (fused.rs:750-770).
+    tiles = 0;
+    tiles_visited = 0;
+    std::vector<uint32_t> diff;
+    for (uint64_t x0 = 0; x0 < nx; x0 += 64) {
+        const uint64_t x1 = std::min(x0 + 64, nx);
+        const uint64_t start = x0 / lnx * local_nj, end = (x1 - 1) / lnx * local_nj + local_nj;
+        tiles += (end + 63) / 64 - start / 64;
+        int64_t first = (int64_t)nj, last = 0;
+        for (uint64_t x = x0; x < x1; x++)
+            if (lo[x] < hi[x]) {
+                first = std::min(first, lo[x]);
+                last = std::max(last, hi[x]);
+            }
+        if (first >= last) continue;
+        const int64_t t0 = first / 64, t1 = (last + 63) / 64;
+        diff.assign((size_t)(t1 - t0 + 1), 0);
+        for (uint64_t x = x0; x < x1; x++)
+            if (lo[x] < hi[x]) {
+                diff[(size_t)(lo[x] / 64 - t0)]++;
+                diff[(size_t)((hi[x] + 63) / 64 - t0)]--;
+            }
+        int64_t run = 0;
+        for (int64_t t = 0; t < t1 - t0; t++) {
+            run += (int32_t)diff[(size_t)t];
+            if (run > 0) tiles_visited++;
+        }
+    }
+
+    // Per-key value statistics (fused.rs:371-384).
+    uint8_t *finite, *has_inf;
+    double *largest, *smallest, *red;
+    int *bad;
+    EF_TRY(c.alloc(finite, nj));
+    EF_TRY(c.alloc(has_inf, nj));
+    EF_TRY(c.alloc(largest, nj));
+    EF_TRY(c.alloc(smallest, nj));
+    EF_TRY(c.alloc(red, 4));
+    EF_TRY(c.zeros(bad, 4));
+    EF_TRY(value_stats(vm, (int64_t)nj, (int64_t)ny, finite, has_inf, largest, smallest, c.stream));
+    const uint32_t weight_kind = pd->weight_kind;
+    const bool standard = prod.row.empty() && weight_kind == 0 && pd->shift == EF_SHIFT_KEPT_MAXIMUM &&
+                          pd->normalizer == EF_NORM_KEPT_GROUPS && pd->scale_kind <= EF_SCALE_OVER &&
+                          std::isfinite(pd->scale) && (pd->scale_kind != EF_SCALE_OVER || pd->scale != 0.0);
+    double *out;
+    EF_TRY(c.alloc(out, nx * ny));
+    bool fast = false;
+    if (standard && nx && nj && ny && ne) {
+        // The MFMA path computes the same rows when no value is special and
+        // the guards can't trip: every score lies within |scale| |q_x| |k_j|
+        // of 0, so every weight is at least lb = exp(-2 |scale| Q K), and
+        // with the sum z in [1, terms] Bounds::hold's tests pass for lb.
+        const double init[4] = {0.0, std::numeric_limits<double>::infinity(), 0.0, 0.0};
+        EF_TRY(c.upload<double>(red, init, 4));
+        EF_TRY(reduce_value_stats(finite, largest, smallest, (int64_t)nj, red, bad, c.stream));
+        EF_TRY(max_norm(qm, (int64_t)nx, (int64_t)ne, (int64_t)ne, 1, red + 2, bad + 1, c.stream));
+        EF_TRY(max_norm(km, (int64_t)nj, (int64_t)ne, 1, (int64_t)nj, red + 3, bad + 2, c.stream));
+        double h[4];
+        int hb[4];
+        EF_TRY(c.download<double>(h, red, 4));
+        EF_TRY(c.download<int>(hb, bad, 4));
+        EF_TRY(c.sync());
+        const double sc = pd->scale_kind == EF_SCALE_TIMES ? std::fabs(pd->scale)
+                          : pd->scale_kind == EF_SCALE_OVER ? 1.0 / std::fabs(pd->scale) : 1.0;
+        const double lb = std::exp(-2.0 * sc * h[2] * h[3] * (1.0 + 1e-6));
+        const double terms = (double)local_nj, normal = 2.0 * DBL_MIN;
+        const bool finite_ok = std::isfinite(h[0] * 2.0 * terms);
+        const bool no_underflow =
+            h[1] == std::numeric_limits<double>::infinity() ||
+            (lb * h[1] >= normal && lb / terms >= normal && lb / terms * h[1] >= normal);
+        fast = !hb[0] && !hb[1] && !hb[2] && finite_ok && no_underflow && std::isfinite(sc * h[2] * h[3]);
+    }
+    if (fast) {
+        // Batches W, queries local X, keys local J, allowed the same in
+        // every batch.
+        const uint64_t nw = nx / lnx;
+        std::vector<int64_t> spans(2 * lnx);
+        for (uint64_t x = 0; x < lnx; x++) {
+            spans[2 * x] = lo[x];
+            spans[2 * x + 1] = hi[x];
+        }
+        int64_t *dspans;
+        double *kmat;
+        EF_TRY(c.alloc(dspans, 2 * lnx));
+        EF_TRY(c.upload<int64_t>(dspans, spans.data(), spans.size()));
+        EF_TRY(c.alloc(kmat, nj * ne));
+        EF_TRY(transpose_f64(km, (int64_t)ne, (int64_t)nj, kmat, c.stream));
+        ef_attention_desc d;
+        std::memset(&d, 0, sizeof d);
+        d.batches = nw;
+        d.queries = lnx;
+        d.keys = local_nj;
+        d.features = ne;
+        d.values = ny;
+        d.scale = pd->scale_kind == EF_SCALE_TIMES ? pd->scale
+                  : pd->scale_kind == EF_SCALE_OVER ? 1.0 / pd->scale : 1.0;
+        d.allowed = dspans;
+        const uint32_t prec = c.options ? c.options->precision : EF_PRECISION_FP64;
+        d.precision = (any32 && prec != EF_PRECISION_FP64) ? prec : EF_PRECISION_FP64;
+        float *rm, *rs;
+        EF_TRY(c.alloc(rm, nx));
+        EF_TRY(c.alloc(rs, nx));
+        uint8_t *scratch;
+        EF_TRY(c.alloc(scratch, attention_scratch_bytes(d)));
+        if (d.precision == EF_PRECISION_FP64) {
+            EF_TRY(attention_fwd(d, qm, kmat, vm, out, rm, rs, nullptr, scratch, c.stream));
+        } else {
+            float *q32, *k32, *v32, *o32;
+            EF_TRY(c.alloc(q32, nx * ne));
+            EF_TRY(c.alloc(k32, nj * ne));
+            EF_TRY(c.alloc(v32, nj * ny));
+            EF_TRY(c.alloc(o32, nx * ny));
+            EF_TRY(narrow_f32(qm, (int64_t)(nx * ne), q32, c.stream));
+            EF_TRY(narrow_f32(kmat, (int64_t)(nj * ne), k32, c.stream));
+            EF_TRY(narrow_f32(vm, (int64_t)(nj * ny), v32, c.stream));
+            EF_TRY(attention_fwd(d, q32, k32, v32, o32, rm, rs, nullptr, scratch, c.stream));
+            EF_TRY(to_f64(Type::Float32, o32, (int64_t)(nx * ny), out, c.stream));
+        }
+    } else {
+        PipeArgs pa{};
+        pa.nx = (int64_t)nx;
+        pa.ne = (int64_t)ne;
+        pa.nj = (int64_t)nj;
+        pa.ny = (int64_t)ny;
+        pa.q = qm;
+        pa.kt = km;
+        pa.v = vm;
+        for (int i = 0; i < 8; i++) {
+            pa.q_cols[i] = qcols[i];
+            pa.k_cols[i] = kcols[i];
+        }
+        pa.n_q_cols = nq_cols;
+        pa.n_k_cols = nk_cols;
+        ef_expr_op *dscore = nullptr, *dweight = nullptr;
+        if (!score.empty()) {
+            EF_TRY(c.alloc(dscore, score.size()));
+            EF_TRY(c.upload<ef_expr_op>(dscore, score.data(), score.size()));
+        }
+        if (weight_kind == 1) {
+            if (!pd->weight || pd->n_weight == 0) {
+                set_error("Weight::Of needs its expression");
+                return EF_INVALID;
+            }
+            EF_TRY(c.alloc(dweight, pd->n_weight));
+            EF_TRY(c.upload<ef_expr_op>(dweight, pd->weight, pd->n_weight));
+        }
+        pa.score = dscore;
+        pa.n_score = (int)score.size();
+        pa.weight = dweight;
+        pa.n_weight = weight_kind == 1 ? (int)pd->n_weight : 0;
+        pa.scale_kind = pd->scale_kind;
+        pa.scale = pd->scale;
+        pa.shifted = pd->shift == EF_SHIFT_KEPT_MAXIMUM;
+        pa.normalizer = pd->normalizer;
+        int64_t *dlo, *dhi;
+        EF_TRY(c.alloc(dlo, nx));
+        EF_TRY(c.alloc(dhi, nx));
+        EF_TRY(c.upload<int64_t>(dlo, lo.data(), nx));
+        EF_TRY(c.upload<int64_t>(dhi, hi.data(), nx));
+        pa.lo = dlo;
+        pa.hi = dhi;
+        pa.finite = finite;
+        pa.has_inf = has_inf;
+        pa.largest = largest;
+        pa.smallest = smallest;
+        pa.out = out;
+        EF_TRY(c.alloc(pa.special, nx * ny));
+        pa.overflowed = bad + 3;
+        EF_TRY(pipeline_rows(pa, c.stream));
+        int ov_flag = 0;
+        EF_TRY(c.download<int>(&ov_flag, bad + 3, 1));
+        EF_TRY(c.sync());
+        if (ov_flag) return EF_DECLINED;
+    }
+
+    // Rows with allowed keys, every y (fused.rs:435-488).
+    std::vector<uint32_t> rows;
+    for (uint64_t x = 0; x < nx; x++)
+        if (lo[x] < hi[x] && nj > 0) rows.push_back((uint32_t)x);
+    const uint64_t nrow = rows.size(), ncell = nrow * ny;
+    uint32_t *drows;
+    EF_TRY(c.alloc(drows, nrow));
+    EF_TRY(c.upload<uint32_t>(drows, rows.data(), nrow));
+    OutKeys okeys{};
+    okeys.n = (int)outs.size();
+    std::vector<uint8_t *> key_out(outs.size());
+    for (size_t i = 0; i < outs.size(); i++) {
+        const uint32_t d = outs[i];
+        const int w = width_of(fields[i].type);
+        EF_TRY(c.alloc(key_out[i], ncell * (w ? w : 8)));
+        okeys.dst[i] = key_out[i];
+        okeys.width[i] = w;
+        okeys.in_x[i] = index_in(xs, d) >= 0;
+        okeys.stride[i] = stride_in(okeys.in_x[i] ? xs : ys, d, extent);
+        okeys.extent[i] = extent[d] ? extent[d] : 1;
+    }
+    double *vals;
+    uint8_t *valid;
+    EF_TRY(c.alloc(vals, ncell));
+    EF_TRY(c.alloc(valid, ncell));
+    EF_TRY(emit((int64_t)ncell, ny, drows, nullptr, nullptr, okeys, out, nullptr, vals, valid, c.stream));
+    std::vector<OutColumn> cols;
+    for (size_t i = 0; i < outs.size(); i++) {
+        OutColumn kc;
+        EF_TRY(fetch_column(c, fields[i].type, key_out[i], nullptr, (int64_t)ncell, fields[i].nullable, kc));
+        cols.push_back(std::move(kc));
+    }
+    OutColumn vc;
+    EF_TRY(fetch_column(c, Type::Float64, vals, nullptr, (int64_t)ncell, true, vc));
+    cols.push_back(std::move(vc));
+    result.push_back(std::move(cols));
+    return EF_OK;
+}
+
+// schema.rs:155-206: the producer's key fields, or v's for the pipeline's
+// own dimensions, then a nullable Float64.
+bool pipeline_schema(const ef_pipeline_desc *pd, const FoldModel &prod, const Relation &q, const Relation &k,
+                     const Relation &v, bool accept32, std::vector<OutField> &fields) {
+    std::vector<OutField> pf;
+    if (!fold_schema(prod, {&q, &k}, nullptr, accept32, pf)) return false;
+    const int vf = pd->values.factor;
+    if (vf < 0 || (size_t)vf >= v.columns() || !factor_type_ok(v.types[vf], accept32)) {
+        set_error("EinFold factors must be Float64");
+        return false;
+    }
+    fields.clear();
+    for (uint32_t i = 0; i < pd->n_outputs; i++) {
+        const uint32_t d = pd->outputs[i];
+        const int o = prod.output_of(d);
+        if (o >= 0) {
+            fields.push_back(pf[o]);
+            continue;
+        }
+        int col = -1;
+        for (uint32_t j = 0; j < pd->values.n_keys; j++)
+            if (pd->values.keys[j].dimension == d) col = (int)pd->values.keys[j].column;
+        if (col < 0 || (size_t)col >= v.columns()) {
+            set_error("a pipeline output is neither produced nor a key of its values");
+            return false;
+        }
+        fields.push_back(OutField{v.types[col], (bool)v.nullable[col]});
+    }
+    fields.push_back(OutField{Type::Float64, true});
+    return true;
+}
+
 }  // namespace ef
 
 using namespace ef;
@@ -1169,6 +1661,53 @@ ef_status ef_fold(const ef_fold_desc *fold, const ef_relation *a, const ef_relat
             return st;
         }
         put_result(std::move(batches), fields, EF_ALGORITHM_DISCRIMINATE, out);
+        return EF_OK;
+    });
+}
+
+ef_status ef_pipeline(const ef_pipeline_desc *pipeline, const ef_relation *q, const ef_relation *k,
+                      const ef_relation *v, const ef_options *options, ef_result *out) {
+    clear_error();
+    if (!out || !pipeline) {
+        set_error("a pipeline and a result are required");
+        return EF_INVALID;
+    }
+    std::memset(out, 0, sizeof *out);
+    return guarded([&]() -> ef_status {
+        FoldModel prod;
+        if (!parse_fold(&pipeline->producer, prod)) return EF_INVALID;
+        const uint32_t n = (uint32_t)prod.equality.size() + pipeline->n_dimensions;
+        for (uint32_t i = 0; i < pipeline->n_outputs; i++)
+            if (pipeline->outputs[i] >= n) {
+                set_error("a pipeline output names a dimension it doesn't have");
+                return EF_INVALID;
+            }
+        for (uint32_t i = 0; i < pipeline->values.n_keys; i++)
+            if (pipeline->values.keys[i].dimension >= n) {
+                set_error("a value key names a dimension the pipeline doesn't have");
+                return EF_INVALID;
+            }
+        if (pipeline->weight_kind > 1 || pipeline->normalizer > EF_NORM_NONE || pipeline->scale_kind > EF_SCALE_OVER) {
+            set_error("unknown weight, scale or normalizer");
+            return EF_INVALID;
+        }
+        Relation rq, rk, rv;
+        if (!read_relation(q, rq) || !read_relation(k, rk) || !read_relation(v, rv)) return EF_INVALID;
+        std::vector<OutField> fields;
+        if (prod.operands.size() != 2) return EF_DECLINED;
+        if (!pipeline_schema(pipeline, prod, rq, rk, rv, options && options->accept_float32, fields))
+            return EF_INVALID;
+        Call c(options);
+        std::vector<std::vector<OutColumn>> batches;
+        uint64_t visited = 0, tiles = 0;
+        ef_status st = pipeline_impl(pipeline, prod, rq, rk, rv, fields, c, batches, visited, tiles);
+        if (st != EF_OK) {
+            (void)hipStreamSynchronize(c.stream);
+            return st;
+        }
+        put_result(std::move(batches), fields, EF_ALGORITHM_FUSED, out);
+        out->tiles_visited = visited;
+        out->tiles = tiles;
         return EF_OK;
     });
 }
