@@ -8,6 +8,9 @@ library's), within 1e-5 relative for Float32 columns computed in BF16x3 or
 FP32 (the stated bar of the mixed form), and NaN exactly where the oracle
 has NaN.
 """
+import json
+import os
+
 import numpy as np
 import pyarrow as pa
 import pytest
@@ -94,11 +97,13 @@ def compare(got, want, tol, relative=False):
     assert err <= tol, err
 
 
+GOLDEN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "known_answers.json")))
+_CMP = {"GreaterOrEqual": GE, "Less": LT, "Greater": GT, "LessOrEqual": LE}
+
+
 @pytest.mark.parametrize("l,batches,conditions,want", [
-    (1024, None, [(GE, 0)], (136, 256)),
-    (1024, None, [(GE, 0), (LT, 64)], (31, 256)),
-    (256, 8, [(GE, 0)], (80, 128)),
-])
+    (c["positions"], c["batches"], [(_CMP[k], o) for k, o in c["mask"]], (c["visited"], c["tiles"]))
+    for c in GOLDEN["tiles_visited"]])
 def test_tile_counts_known_answers(cuda, l, batches, conditions, want):
     """pipeline_vs_sql.rs:734-760: tiles visited of all under causal and
     band masks."""
@@ -135,6 +140,17 @@ def test_float32_columns_mixed_precision(cuda, precision, tol):
                       R.pipeline_schema(p, [f64(q).schema, f64(k).schema, f64(v).schema]))
     got = kernels.pipeline(p, q, k, v, kernels.Options(accept_float32=True, precision=precision))
     compare(got, want, tol, relative=True)
+
+
+def test_batches_not_a_multiple_of_the_tile(cuda):
+    """pipeline_vs_sql.rs:757-760: batches of 100 positions; a tile holding
+    rows of two batches counts against both (at most 13 x 4 tiles)."""
+    rng = np.random.default_rng(29)
+    q, k, v = attention_tables(rng, 100, 8, 4, 8, q_scale=0.3)
+    p = attention_pipeline([(GE, 0)], True)
+    got, want = both(p, q, k, v)
+    assert got.tiles <= 13 * 4 and got.tiles_visited <= got.tiles
+    compare(got, want, 1e-9 * np.max(np.abs(v.column("v").to_numpy())))
 
 
 def test_float32_columns_need_the_option(cuda):

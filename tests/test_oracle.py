@@ -2,7 +2,9 @@
 closed forms, an independent nested-loop SQL evaluation, and exact
 metamorphic relations, chosen so that a dropped term, a wrong index or a
 transposed operand fails one of them."""
+import json
 import math
+import os
 
 import numpy as np
 import pyarrow as pa
@@ -14,13 +16,16 @@ from einfold_amd import data, ir
 
 from sqlsem import fold_rows, within_bound
 
+GOLDEN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "known_answers.json")))
+
 EQ = ir.KeyEquality.EQUAL
 
 
 # ------------------------------------------------------------ generator
 def test_splitmix64_known_answers():
-    """crates/einfold-testkit/src/rng.rs:65-78."""
-    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    """crates/einfold-testkit/src/rng.rs:65-78 (tests/golden/known_answers.json)."""
+    want = [int(x, 16) for x in GOLDEN["splitmix64"]["outputs"]]
+    assert len(want) == 4
     assert [int(x) for x in oracle.splitmix_next(0, 0, 4)] == want
     assert [int(x) for x in data.next_u64(0, 0, 4)] == want
 
@@ -247,11 +252,13 @@ def _attention_arrow(l, batches, q_fill, k_fill, v_fill):
     return (table(pre + [l, 8], q_fill), table(pre + [l, 8], k_fill), table(pre + [l], v_fill))
 
 
-@pytest.mark.parametrize("l,batches,conditions,want", [
-    (1024, None, [(ir.Comparison.GREATER_OR_EQUAL, 0)], (136, 256)),
-    (1024, None, [(ir.Comparison.GREATER_OR_EQUAL, 0), (ir.Comparison.LESS, 64)], (31, 256)),
-    (256, 8, [(ir.Comparison.GREATER_OR_EQUAL, 0)], (80, 128)),
-])
+_CMP = {"GreaterOrEqual": ir.Comparison.GREATER_OR_EQUAL, "Less": ir.Comparison.LESS,
+        "Greater": ir.Comparison.GREATER, "LessOrEqual": ir.Comparison.LESS_OR_EQUAL}
+TILE_CASES = [(c["positions"], c["batches"], [(_CMP[k], o) for k, o in c["mask"]], (c["visited"], c["tiles"]))
+              for c in GOLDEN["tiles_visited"]]
+
+
+@pytest.mark.parametrize("l,batches,conditions,want", TILE_CASES)
 def test_tile_counts_known_answers(l, batches, conditions, want):
     """pipeline_vs_sql.rs:734-760, and every average of v = 1 is 1."""
     rng = np.random.default_rng(7)
@@ -267,35 +274,19 @@ def test_tile_counts_known_answers(l, batches, conditions, want):
 
 
 def test_bounds_known_cases():
-    """fused.rs:1116-1177."""
-    hold = lambda *a: bool(oracle.lib().ef_oracle_bounds_hold(*a))
+    """fused.rs:1116-1177: every Bounds::hold case einfold's tests hold
+    (tests/golden/known_answers.json)."""
     tiny = np.finfo(float).smallest_subnormal
-
-    def hb(heaviest, lightest, largest, smallest, z, results, special=None, has_z=True):
-        res = np.array(results, np.float64)
-        sp = np.array(special if special is not None else [0] * len(results), np.uint8)
-        return hold(heaviest, lightest, largest, smallest, 3, int(has_z), z if z is not None else 0.0,
-                    oracle.ptr(res), oracle.ptr(sp), len(res))
-    # an_exact_zero_of_normal_terms_holds
-    assert hb(1.0, 0.25, 2.0, 0.5, 1.5, [0.0, 1.0])
-    assert hb(1.0, 0.25, 2.0, 0.5, None, [0.0], has_z=False)
-    assert hb(1.0, 0.25, 0.0, math.inf, 1.5, [0.0])
-    # an_exact_zero_of_terms_that_may_underflow_declines
-    assert not hb(1.0, 1.0, 1.0, 1e-300, 1e10, [0.0])
-    assert not hb(1.0, 1e-300, 1.0, 1e-10, None, [0.0], has_z=False)
-    # ordinary / negligible underflow
-    assert hb(1.0, tiny, 1e5, 1e5, 1.5, [0.3, -2.0])
-    assert hb(1.0, tiny, 1e5, 1e5, None, [0.3, -2.0], has_z=False)
-    assert hb(1.1e188, tiny, 1.0, 1.0, 1.1e188, [0.4])
-    # products / quotients that round to zero, overflow
-    assert not hb(1e-323, tiny, 0.125, 0.125, 2e-323, [0.0])
-    assert not hb(2.0, tiny, 1e300, 1e300, 2.0, [2.5e-24])
-    assert not hb(1.0, tiny, 1e100, 1e100, 1e-309, [1e100])
-    assert hb(1.0, tiny, 1e100, 1e100, None, [1e100], has_z=False)
-    assert not hb(1.0, tiny, 1e308, 1e308, None, [1e308], has_z=False)
-    # a_special_result_is_sqls
-    assert hb(1.0, tiny, 1.0, 1.0, 1.0, [0.0, math.inf], [1, 1])
-    assert not hb(1.0, tiny, 1.0, 1.0, 1.0, [0.0], [0])
+    cases = GOLDEN["bounds_hold"]
+    assert len(cases) >= 12
+    for c in cases:
+        res = np.array(c["results"], np.float64)
+        sp = np.array(c["special"], np.uint8)
+        lightest = tiny if c["lightest"] == "subnormal_min" else c["lightest"]
+        got = bool(oracle.lib().ef_oracle_bounds_hold(
+            c["heaviest"], lightest, c["largest"], c["smallest"], c["terms"], int(c["z"] is not None),
+            c["z"] if c["z"] is not None else 0.0, oracle.ptr(res), oracle.ptr(sp), len(res)))
+        assert got == c["holds"], c
 
 
 def test_expression_known_answers():
