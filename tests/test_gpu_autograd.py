@@ -1,0 +1,107 @@
+"""The backward kernels through torch.autograd, checked three ways:
+
+* FP64 central differences (eps = 1e-3) of SUM(out * g) on 256 random
+  entries of each of q, k and v, at 4 heads x T = 2048 x d = 64;
+  the saved row statistics are float32, so the FP64 form's gradients are
+  within about 1e-7 relative, not to the last bit;
+* torch float64 autograd of the same attention on the CPU;
+* the left-joined gradient folds that ddx writes for dV
+  (docs/spikes/s31-ddx-gradient-steps): dV(h, tk, d) = SUM over tq of
+  P(h, tq, tk) * dO(h, tq, d), left-joined from the v table with 0.0 for
+  keys no query reaches, run by this build's dense fold operator.
+"""
+import numpy as np
+import pyarrow as pa
+import pytest
+import torch
+
+from einfold_amd import autograd, data, ir, kernels
+
+pytestmark = pytest.mark.gpu
+
+
+def _tables(h, t, d, seed):
+    q, k, v = data.attention_tables(h, t, d, seed=seed)
+    g = data.normal(seed, 4, h * t * d).reshape(h, t, d)
+    return q, k, v, g
+
+
+def _loss(q, k, v, g, scale, precision):
+    return (autograd.fused_attention(q, k, v, scale, True, precision) * g).sum()
+
+
+def test_central_differences_fp64(cuda):
+    """4 heads, T = 2048, d = 64 (the gradient workload): the FP64 backward
+    against (L(x + eps) - L(x - eps)) / 2 eps at 256 entries per tensor."""
+    h, t, d, eps, scale = 4, 2048, 64, 1e-3, 0.125
+    q, k, v, g = (torch.from_numpy(np.ascontiguousarray(x)).to(cuda) for x in _tables(h, t, d, 5))
+    qg, kg, vg = (x.clone().requires_grad_(True) for x in (q, k, v))
+    _loss(qg, kg, vg, g, scale, "fp64").backward()
+    rng = np.random.default_rng(0)
+    for name, base, grad in (("q", q, qg.grad), ("k", k, kg.grad), ("v", v, vg.grad)):
+        idx = rng.choice(base.numel(), 256, replace=False)
+        fd = np.empty(256)
+        with torch.no_grad():
+            for n, i in enumerate(idx):
+                args = {"q": q, "k": k, "v": v}
+                plus, minus = base.clone(), base.clone()
+                plus.view(-1)[i] += eps
+                minus.view(-1)[i] -= eps
+                lp = _loss(*(plus if nm == name else args[nm] for nm in ("q", "k", "v")), g, scale, "fp64")
+                lm = _loss(*(minus if nm == name else args[nm] for nm in ("q", "k", "v")), g, scale, "fp64")
+                fd[n] = (lp - lm).item() / (2 * eps)
+        got = grad.view(-1)[torch.from_numpy(idx).to(cuda)].cpu().numpy()
+        err = np.linalg.norm(got - fd) / np.linalg.norm(fd)
+        assert err < 1e-5, (name, err)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("fp32", 1e-5), ("bf16x3", 3e-5)])
+def test_autograd_against_torch_float64(cuda, precision, tol):
+    h, t, d, scale = 2, 384, 64, 0.125
+    q, k, v, g = _tables(h, t, d, 6)
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    xs = [torch.from_numpy(np.ascontiguousarray(x)).to(dt).to(cuda).requires_grad_(True) for x in (q, k, v)]
+    gt = torch.from_numpy(g).to(dt).to(cuda)
+    _loss(*xs, gt, scale, precision).backward()
+    r = [torch.from_numpy(np.ascontiguousarray(x)).double().requires_grad_(True) for x in (q, k, v)]
+    s = scale * r[0] @ r[1].transpose(1, 2)
+    mask = torch.ones(t, t, dtype=torch.bool).tril()
+    p = torch.softmax(s.masked_fill(~mask, float("-inf")), dim=-1)
+    ((p @ r[2]) * torch.from_numpy(g)).sum().backward()
+    for x, want in zip(xs, r):
+        got = x.grad.double().cpu()
+        err = float(torch.linalg.vector_norm(got - want.grad) / torch.linalg.vector_norm(want.grad))
+        assert err < tol, err
+
+
+def test_dv_equals_the_left_joined_gradient_fold(cuda):
+    """dV from the backward kernel (FP64) against ddx's gradient SQL for dV
+    run as a dense fold from the v table's domain with missing 0.0."""
+    h, t, d, scale = 2, 256, 32, 0.125
+    q, k, v, g = _tables(h, t, d, 7)
+    xs = [torch.from_numpy(np.ascontiguousarray(x)).to(cuda).requires_grad_(True) for x in (q, k, v)]
+    _loss(*xs, torch.from_numpy(g).to(cuda), scale, "fp64").backward()
+    dv = xs[2].grad.cpu().numpy()
+    s = scale * np.einsum("htd,hsd->hts", q, k)
+    s = np.where(np.tril(np.ones((t, t), bool)), s, -np.inf)
+    p = np.exp(s - s.max(-1, keepdims=True))
+    p /= p.sum(-1, keepdims=True)
+    hh, tq, tk = np.nonzero(p > 0)
+    ptab = pa.table({"h": hh.astype(np.int64), "tq": tq.astype(np.int64), "tk": tk.astype(np.int64),
+                     "w": p[hh, tq, tk]})
+    gh, gt, gd = np.meshgrid(np.arange(h), np.arange(t), np.arange(d), indexing="ij")
+    gtab = pa.table({"h": gh.reshape(-1), "tq": gt.reshape(-1), "d": gd.reshape(-1), "v": g.reshape(-1)})
+    vtab = pa.table({"h": gh.reshape(-1), "tk": gt.reshape(-1), "d": gd.reshape(-1)})
+    EQ = ir.KeyEquality.EQUAL
+    # dims 0 = h, 1 = tq, 2 = tk, 3 = d; GROUP BY h, tk, d from the domain v.
+    fold = ir.Fold([EQ] * 4, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1), ir.Key(2, 2)), 3),
+                              ir.Operand((ir.Key(0, 0), ir.Key(1, 1), ir.Key(3, 2)), 3)],
+                   [ir.Output(0, ir.DomainKey(0)), ir.Output(2, ir.DomainKey(1)), ir.Output(3, ir.DomainKey(2))],
+                   domain=ir.Domain([0, 1, 2], 0.0))
+    out = kernels.fold_dense(fold, ptab, gtab, vtab)
+    assert out is not None
+    tab = pa.Table.from_batches(out)
+    got = tab.column(3).to_numpy().reshape(h, t, d)
+    assert np.array_equal(tab.column(1).to_numpy().reshape(h, t, d)[0, :, 0], np.arange(t))
+    err = np.max(np.abs(got - dv)) / np.max(np.abs(dv))
+    assert err < 1e-6, err
