@@ -30,6 +30,10 @@ H, T, D = 8, 4096, 64
 SCALE = 1.0 / 8.0
 PEAK_BF16 = 2.5e15       # MI355X dense BF16 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12
+# HBM bytes one launch of the dK/dV kernel (BF16x3, d = 64) moves at this
+# workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors a
+# 256 MiB device copy reads back (profiles/attention_causal_h8_t4096_d64_hbm_traffic.md).
+DKDV_TRAFFIC_BYTES = (191544.2 * 2.0 + 16400.0) * 1024
 
 
 def pairs_per_head(t: int) -> int:
@@ -42,6 +46,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the dense fold and ranking measurements")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one plain 256 MiB device copy first (a known-size kernel for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -54,6 +62,12 @@ def main():
 
     dev = torch.device("cuda:0")
     check(lib().ef_init(0))
+    if args.copy_probe:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     q64, k64, v64 = data.attention_tables(H, T, D, seed=3)
     g64 = data.normal(3, 4, H * T * D).reshape(H, T, D)
     to = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x)).to(dt).to(dev)
@@ -141,6 +155,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
+    secondary = None
+    if not args.no_secondary:
+        secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev)}
 
     line = {
         "metric": "fold GFLOP/s + attention fwd+bwd ms: mixed vs plain FP32, 1 MI355X",
@@ -164,12 +181,100 @@ def main():
                      "achieved": round(achieved / 1e12, 2), "peak": PEAK_BF16 / 1e12,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16, 4),
                      "issued_frac": round(3 * achieved / PEAK_BF16, 4),
-                     "avg_ms": round(kv_ms, 4), "traffic": None,
+                     "avg_ms": round(kv_ms, 4), "traffic": round(DKDV_TRAFFIC_BYTES),
+                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated",
                      "fp32_kernel": {"avg_ms": round(fp32_kv_ms, 4),
                                      "frac_of_fp32_peak": round(flop_kv / (fp32_kv_ms * 1e-3) / PEAK_FP32, 4)}},
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
+
+
+def _events(fn, reps, stream_fn=None):
+    """Average ms of fn() over reps, by HIP events on torch's stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def dense_fold_line(dev, with_cpu: bool):
+    """The plain dense matrix-product fold: a(i, j, v), b(j, k, v) over every
+    cell of a 256 x 256 grid, Int64 keys, v ~ U[-1, 1]; 2 * 256^3 FLOP.  The
+    contraction kernel alone (device-resident), the operator from Arrow
+    batches to Arrow batches, and the oracle's C restatement of dense.rs's
+    contraction on one core."""
+    import torch
+    from einfold_amd import data, device, ir, kernels
+    n = 256
+    a, b = data.grid_matmul(n)
+    A = torch.from_numpy(a.column("v").to_numpy().reshape(n, n).copy()).to(dev)
+    B = torch.from_numpy(b.column("v").to_numpy().reshape(1, n, n).copy()).to(dev)
+    held = torch.full((n, n), 2, dtype=torch.int32, device=dev)
+    flop = 2.0 * n ** 3
+    kms = _events(lambda: device.dense_accumulate_f64(A, held, B, n), 50)
+    EQ = ir.KeyEquality.EQUAL
+    fold = ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                              ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)], [ir.Output(0), ir.Output(2)])
+    kernels.fold_dense(fold, a, b)
+    walls = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        out = kernels.fold_dense(fold, a, b)
+        walls.append(time.perf_counter() - t0)
+    assert out is not None and sum(x.num_rows for x in out) == n * n
+    line = {"workload": "dense fold 256^3 grid (a.i, b.k, SUM(a.v*b.v)), FP64, bit-exact reference order",
+            "kernel_ms": round(kms, 4), "kernel_gflops": round(flop / kms / 1e6, 1),
+            "arrow_to_arrow_ms": round(float(np.median(walls)) * 1e3, 3),
+            "bound": "fp64 valu (exact order; no FMA contraction)",
+            "frac_of_fp64_peak": round(flop / (kms * 1e-3) / 78.6e12, 4)}
+    if with_cpu:
+        import oracle
+        Ah = a.column("v").to_numpy().reshape(n, n)
+        Bh = b.column("v").to_numpy().reshape(n, n)
+        hh = np.full((n, n), 2, np.uint8)
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            oracle.dense_accumulate(Ah, hh, Bh, n, n, n, n)
+        dt = (time.perf_counter() - t0) / reps
+        line["cpu_port_ms"] = round(dt * 1e3, 3)
+        line["cpu_port_cores"] = 1
+    return line
+
+
+def ranking_line(dev):
+    """Radix ranking of the sorting path's join key: j over both operands,
+    8M UInt64 hashes of Zipf(1.1) ids over 1M ids.  Algorithmic traffic 12 B
+    per key (read the 8-byte word, write the 4-byte rank)."""
+    import ctypes
+    import torch
+    from einfold_amd import data
+    from einfold_amd._lib import lib, check
+    rows = 4 << 20
+    a_j = data.hash64(data.zipf_ids(5, 1, rows, 1 << 20, 1.1), 0x6A)
+    b_j = data.hash64(data.zipf_ids(5, 2, rows, 1 << 20, 1.1), 0x6A)
+    words = torch.from_numpy(np.concatenate([a_j, b_j]).view(np.int64)).to(dev)
+    n = words.numel()
+    rank = torch.empty(n, dtype=torch.int32, device=dev)
+    first = torch.empty(n, dtype=torch.int32, device=dev)
+    scratch = torch.empty(lib().ef_rank_scratch_bytes(n), dtype=torch.uint8, device=dev)
+    k = ctypes.c_size_t(0)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    run = lambda: check(lib().ef_rank_u64(P(words), n, P(rank), P(first), ctypes.byref(k), P(scratch), stream))
+    ms = _events(run, 10)
+    gbs = 12.0 * n / (ms * 1e-3) / 1e9
+    return {"workload": "radix ranking of 8M UInt64 join keys (Zipf 1.1 over 1M ids)", "keys": n,
+            "distinct": int(k.value), "ms": round(ms, 4), "gb_per_s": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / 8000.0, 4)}
 
 
 def cpu_baseline():
