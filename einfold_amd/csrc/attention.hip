@@ -11,9 +11,13 @@
 // to whole tiles with zeros, so each tile is staged into LDS by plain
 // 16-byte copies.
 //
-// Scheduling: workgroups are persistent; each takes (batch, tile) items
-// from a counter, the heaviest first under a causal mask, so the
-// triangle's long rows don't finish last.  Each item is computed whole by
+// Scheduling: workgroups are persistent; each takes (batch, block, part)
+// items from a counter, the heaviest first.  Under a causal or other
+// varying mask a block's range of streamed tiles is cut in two parts,
+// each computed by one workgroup into a partial result, and a small
+// kernel combines the two in a fixed order: the triangle's long rows are
+// then no longer than twice the short ones' work, so no CU idles while
+// one finishes them, and the bits never depend on who took which part.  Each item is computed whole by
 // one workgroup, keys in order, so the bits never depend on which
 // workgroup took it.
 #include <hip/hip_runtime.h>
@@ -27,8 +31,11 @@ namespace ef {
 
 namespace {
 
-constexpr int BN = 64;    // rows of a streamed tile (keys in fwd/dq, queries in dk/dv)
-constexpr int BW = 128;   // rows a workgroup owns (4 waves x 32)
+constexpr int BN = 64;          // rows of a streamed tile (keys in fwd/dq, queries in dk/dv)
+constexpr int WAVES = 4;        // waves per workgroup, 32 owned rows each
+constexpr int THREADS = 64 * WAVES;
+constexpr int BW = 32 * WAVES;  // rows a workgroup owns
+constexpr int PAD = 128;        // prepared operands are padded to whole 128-row blocks
 constexpr double LOG2E = 1.4426950408889634;   // cast to the precision's scalar
 constexpr double LN2 = 0.6931471805599453;
 
@@ -135,7 +142,7 @@ __device__ __forceinline__ void stage(E *lds, int lpitch, int lplane, const E *g
     const int per_row = cols / CE;
     const int per_plane = rows * per_row;
     const int total = planes * per_plane;
-    for (int k = threadIdx.x; k < total; k += 256) {
+    for (int k = threadIdx.x; k < total; k += THREADS) {
         const int p = k / per_plane, rem = k % per_plane;
         const int r = rem / per_row, c = (rem % per_row) * CE;
         const u32x4 v = *reinterpret_cast<const u32x4 *>(g + p * gplane + r * gstride + c);
@@ -163,7 +170,7 @@ __device__ __forceinline__ void group_range(const Span (&sp)[2], int64_t *red_lo
     __syncthreads();
     first = red_lo[0];
     last = red_hi[0];
-    for (int i = 1; i < 4; i++) {
+    for (int i = 1; i < WAVES; i++) {
         first = red_lo[i] < first ? red_lo[i] : first;
         last = red_hi[i] > last ? red_hi[i] : last;
     }
@@ -183,11 +190,12 @@ struct FwdArgs {
     Mask mask;
     int64_t tq, tk, tkp, batches;
     double scale;
-    int items, reverse;
+    int items, reverse, parts, pad;
+    void *po, *pm, *pl;           // parts == 2: each part's unnormalized sums, max and sum
 };
 
 // The K and V^T tiles of one 64-key step, as 16-byte chunks: thread t
-// moves chunks t, t + 256, ...; the first KN of each thread's are K's.
+// moves chunks t, t + THREADS, ...; the first KN of each thread's are K's.
 template <int P, int D> struct FwdTiles {
     typedef typename Elem<P>::T E;
     static constexpr int PL = Elem<P>::planes;
@@ -196,13 +204,13 @@ template <int P, int D> struct FwdTiles {
     static constexpr int KPLANE = BN * KP, VPLANE = D * VP;
     static constexpr int KCH = PL * BN * (D / CE);     // K chunks
     static constexpr int VCH = PL * D * (BN / CE);     // V^T chunks
-    static constexpr int KN = KCH / 256, VN = VCH / 256, N = KN + VN;
-    static constexpr bool prefetch = N <= 8;           // hold the next tile in registers
+    static constexpr int KN = KCH / THREADS, VN = VCH / THREADS, N = KN + VN;
+    static constexpr bool prefetch = N <= 16;          // hold the next tile in registers
     static constexpr int BUF = PL * (KPLANE + VPLANE); // elements per LDS buffer
 };
 
 template <int P, int D>
-__global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
+__global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
     typedef typename Input<P>::T In;
     typedef typename Acc<P>::type AccV;
     typedef typename Acc<P>::scalar S;
@@ -211,8 +219,8 @@ __global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
     constexpr int KC = D / 32, DB = D / 16, NP = Parts<P>::n;
     constexpr int NBUF = F::prefetch ? 2 : 1;
     __shared__ __attribute__((aligned(16))) E lds[NBUF * F::BUF];
-    __shared__ int64_t red_lo[4], red_hi[4];
-    __shared__ int flags[2][4];   // per buffer, per wave: bit 0 any, bit 1 full
+    __shared__ int64_t red_lo[WAVES], red_hi[WAVES];
+    __shared__ int flags[2][WAVES];   // per buffer, per wave: bit 0 any, bit 1 full
     __shared__ int slot;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
@@ -228,12 +236,12 @@ __global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
 #pragma unroll
     for (int i = 0; i < F::N; i++) {
         if (i < F::KN) {
-            const int k = tid + 256 * i, per_row = D / F::CE, per_plane = BN * per_row;
+            const int k = tid + THREADS * i, per_row = D / F::CE, per_plane = BN * per_row;
             const int pl = k / per_plane, r = (k % per_plane) / per_row, col = (k % per_row) * F::CE;
             goff[i] = (int)(pl * kplane + (int64_t)r * D + col);
             loff[i] = pl * F::KPLANE + r * F::KP + col;
         } else {
-            const int k = tid + 256 * (i - F::KN), per_row = BN / F::CE, per_plane = D * per_row;
+            const int k = tid + THREADS * (i - F::KN), per_row = BN / F::CE, per_plane = D * per_row;
             const int pl = k / per_plane, r = (k % per_plane) / per_row, col = (k % per_row) * F::CE;
             goff[i] = (int)(pl * vplane + (int64_t)r * a.tkp + col);
             loff[i] = F::PL * F::KPLANE + pl * F::VPLANE + r * F::VP + col;
@@ -242,8 +250,10 @@ __global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
     for (;;) {
         const int item = take_item(a.ctr, &slot);
         if (item >= a.items) break;
-        const int64_t b = item % a.batches;
-        const int64_t qt = a.reverse ? nqt - 1 - item / a.batches : item / a.batches;
+        const int part = item % a.parts;
+        const int64_t blk = item / a.parts;
+        const int64_t b = blk % a.batches;
+        const int64_t qt = a.reverse ? nqt - 1 - blk / a.batches : blk / a.batches;
         const int64_t q0 = qt * BW;
         const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
         const E *K = static_cast<const E *>(a.kr) + b * a.tkp * D;
@@ -266,7 +276,13 @@ __global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
         }
         int64_t first, last;
         group_range(sp, red_lo, red_hi, first, last);
-        const int64_t jbeg = first < last ? first - first % BN : last;
+        int64_t jbeg = first < last ? first - first % BN : last;
+        if (a.parts == 2) {
+            // Part 0 takes the first half of the key tiles, part 1 the rest.
+            const int64_t cut = jbeg + ((last - jbeg + 2 * BN - 1) / (2 * BN)) * BN;
+            if (part == 0) last = cut < last ? cut : last;
+            else jbeg = cut < last ? cut : last;
+        }
 
         S m[2], l[2];
         AccV o[2][DB];
@@ -320,11 +336,15 @@ __global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
                 stage<E>(lds + F::PL * F::KPLANE, F::VP, F::VPLANE, VT + j0, a.tkp, vplane, D, BN, F::PL);
                 __syncthreads();
             }
-            const int f[4] = {flags[cur][0], flags[cur][1], flags[cur][2], flags[cur][3]};
-            const bool full_wg = (f[0] & f[1] & f[2] & f[3] & 2) != 0;
-            // Tiles in the reference's 64 x 64 units: each half of the
-            // workgroup's 128 queries counts the tiles it sees.
-            if (tid == 0) visited += (unsigned)(((f[0] | f[1]) & 1) != 0) + (unsigned)(((f[2] | f[3]) & 1) != 0);
+            int f_and = 3;
+#pragma unroll
+            for (int i = 0; i < WAVES; i++) f_and &= flags[cur][i];
+            const bool full_wg = (f_and & 2) != 0;
+            // Tiles in the reference's 64 x 64 units: each 64 queries (two
+            // waves) of the workgroup count the tiles they see.
+            if (tid == 0)
+#pragma unroll
+                for (int h = 0; h < WAVES / 2; h++) visited += (unsigned)(((flags[cur][2 * h] | flags[cur][2 * h + 1]) & 1) != 0);
             const E *ks = lds + (F::prefetch ? cur * F::BUF : 0);
             const E *vt = ks + F::PL * F::KPLANE;
 
@@ -422,6 +442,23 @@ __global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
         }
         // out[x][d] for d = db*16 + crow(g, i): the weighted sums over the
         // normalizer (a query with no keys has l = 0 and gets 0/0 = NaN).
+        if (a.parts == 2) {
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++) {
+                if (x[qb] >= a.tq) continue;
+                const int64_t r = (part * a.batches + b) * a.tq + x[qb];
+                S *PO = static_cast<S *>(a.po) + r * D;
+#pragma unroll
+                for (int db = 0; db < DB; db++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) PO[db * 16 + crow<P>(g, i)] = o[qb][db][i];
+                if (g == 0) {
+                    static_cast<S *>(a.pm)[r] = m[qb];
+                    static_cast<S *>(a.pl)[r] = l[qb];
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int qb = 0; qb < 2; qb++) {
             if (x[qb] >= a.tq) continue;
@@ -441,6 +478,33 @@ __global__ __launch_bounds__(256, 1) void attn_fwd_kernel(FwdArgs a) {
 #undef EF_WAVE_FLAGS
 #undef EF_LOAD_TILE
 #undef EF_STORE_TILE
+
+// The two parts of each query's keys combined, part 0 then part 1: the
+// sums rescaled to the larger max (log2 units), then normalized.
+template <class S, class In>
+__global__ __launch_bounds__(256) void attn_merge_kernel(const S *po, const S *pm, const S *pl, int64_t rows, int d,
+                                                         In *out, float *row_max, float *row_sum) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * d) return;
+    const int64_t r = i / d;
+    const S m0 = pm[r], m1 = pm[rows + r];
+    const S mx = m0 > m1 ? m0 : m1;
+    const S a0 = m0 == -INFINITY ? S(0) : fast_exp2(m0 - mx);
+    const S a1 = m1 == -INFINITY ? S(0) : fast_exp2(m1 - mx);
+    const S l = pl[r] * a0 + pl[rows + r] * a1;
+    out[i] = (In)((po[i] * a0 + po[rows * d + i] * a1) / l);
+    if (i % d == 0) {
+        row_max[r] = (float)(mx * (S)LN2);
+        row_sum[r] = (float)l;
+    }
+}
+
+// out = (p0 + p1) * scale, part 0 first.
+template <class S, class In>
+__global__ __launch_bounds__(256) void attn_sum_kernel(const S *p, int64_t n, double scale, In *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (In)((p[i] + p[n + i]) * (S)scale);
+}
 
 // Any shape, one thread per (query, block of 8 value columns), keys in
 // order: the reference's algorithm with a tile of one row.  Float64 or
@@ -503,7 +567,8 @@ struct BwdArgs {
     Mask mask;
     int64_t tq, tk, tqp, tkp, batches;
     double scale;
-    int items, reverse, causal_only, pad;
+    int items, reverse, causal_only, parts;
+    void *pdk, *pdv, *pdq;             // parts == 2: each part's unscaled sums
 };
 
 // delta[b][x] = sum_d dout[b][x][d] * out[b][x][d], a fixed-order tree.
@@ -521,7 +586,7 @@ __global__ __launch_bounds__(256) void attn_delta_kernel(const In *out, const In
 // dK and dV for 128 keys per workgroup (32 per wave), visiting the query
 // tiles that may see them.
 template <int P, int D>
-__global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
+__global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     typedef typename Input<P>::T In;
     typedef typename Acc<P>::type AccV;
     typedef typename Acc<P>::scalar S;
@@ -534,7 +599,7 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     __shared__ __attribute__((aligned(16))) E qtr[PL * TPLANE];
     __shared__ __attribute__((aligned(16))) E dos[PL * RPLANE];
     __shared__ __attribute__((aligned(16))) E dotr[PL * TPLANE];
-    __shared__ float sm[BN], sl[BN], sd[BN];
+    __shared__ float sm[BN], sd[BN];   // per query: m log2(e) + log2(l), and delta
     __shared__ int64_t slo[BN], shi[BN];
     __shared__ int slot;
 
@@ -548,9 +613,12 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     for (;;) {
         const int item = take_item(a.ctr, &slot);
         if (item >= a.items) break;
-        const int64_t b = item % a.batches;
-        const int64_t kt = a.reverse ? nkt - 1 - item / a.batches : item / a.batches;
+        const int part = item % a.parts;
+        const int64_t blk = item / a.parts;
+        const int64_t b = blk % a.batches;
+        const int64_t kt = a.reverse ? nkt - 1 - blk / a.batches : blk / a.batches;
         const int64_t k0 = kt * BW;
+        const int64_t kvalid = (k0 + BW < a.tk ? k0 + BW : a.tk);   // keys past tk are padding
         const In *K = static_cast<const In *>(a.k) + b * a.tk * D;
         const In *V = static_cast<const In *>(a.v) + b * a.tk * D;
         const E *QR = static_cast<const E *>(a.qr) + b * a.tqp * D;
@@ -583,29 +651,32 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
             for (int i = 0; i < DB; i++) { dk[kb][i] = AccV{0, 0, 0, 0}; dv[kb][i] = AccV{0, 0, 0, 0}; }
 
         // Under a causal mask alone, query x sees key j when j <= x + offset.
-        int64_t qt_begin = 0;
+        int64_t qt_begin = 0, qt_end = nqt;
         if (a.causal_only) {
             const int64_t xmin = k0 - a.mask.offset;
             qt_begin = xmin > 0 ? xmin / BN : 0;
+            qt_begin = qt_begin < nqt ? qt_begin : nqt;
         }
-        for (int64_t qt = qt_begin; qt < nqt; qt++) {
+        if (a.parts == 2) {
+            const int64_t cut = qt_begin + (qt_end - qt_begin + 1) / 2;
+            if (part == 0) qt_end = cut;
+            else qt_begin = cut;
+        }
+        for (int64_t qt = qt_begin; qt < qt_end; qt++) {
             const int64_t q0 = qt * BN;
+            bool any = false, full = true;
             if (tid < BN) {
                 const int64_t xq = q0 + tid;
                 const Span sp = query_span(a.mask, xq);
                 slo[tid] = sp.lo;
                 shi[tid] = sp.hi;
-                sm[tid] = xq < a.tq ? RM[xq] : 0.f;
-                sl[tid] = xq < a.tq ? RS[xq] : 1.f;
+                sm[tid] = xq < a.tq ? RM[xq] * (float)LOG2E + __log2f(RS[xq]) : 0.f;
                 sd[tid] = xq < a.tq ? DL[xq] : 0.f;
-            }
-            __syncthreads();
-            bool any = false;
-            if (tid < BN) {
-                const int64_t kend = k0 + BW < a.tk ? k0 + BW : a.tk;
-                any = slo[tid] < shi[tid] && slo[tid] < kend && shi[tid] > k0;
+                any = sp.lo < sp.hi && sp.lo < kvalid && sp.hi > k0;
+                full = xq >= a.tq || (sp.lo <= k0 && sp.hi >= kvalid);
             }
             if (!__syncthreads_or(any)) continue;
+            const bool full_tile = __syncthreads_and(full);
             stage<E>(qs, RP, RPLANE, QR + q0 * D, D, rplane, BN, D, PL);
             stage<E>(qtr, TP, TPLANE, QT + q0, a.tqp, tplane, D, BN, PL);
             stage<E>(dos, RP, RPLANE, DR + q0 * D, D, rplane, BN, D, PL);
@@ -638,20 +709,23 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                             }
                     }
                 }
-                // P = exp(scale s - m) / l, dS = P (dP - delta), 0 where masked.
+                // P = exp2(scale s log2(e) - m log2(e) - log2(l)), dS =
+                // P (dP - delta), 0 where masked (no test on a full tile).
                 S p[2][2][4], ds[2][2][4];
 #pragma unroll
                 for (int qq = 0; qq < 2; qq++)
 #pragma unroll
-                    for (int kb = 0; kb < 2; kb++)
+                    for (int i = 0; i < 4; i++) {
+                        const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
+                        const S mr = (S)sm[r], dr = (S)sd[r];
 #pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
-                            const bool ok = kj[kb] >= slo[r] && kj[kb] < shi[r];
-                            const S pv = ok ? fast_exp2(s[qq][kb][i] * scale2 - (S)sm[r] * (S)LOG2E) / (S)sl[r] : S(0);
+                        for (int kb = 0; kb < 2; kb++) {
+                            S pv = fast_exp2(s[qq][kb][i] * scale2 - mr);
+                            if (!full_tile && !(kj[kb] >= slo[r] && kj[kb] < shi[r])) pv = S(0);
                             p[qq][kb][i] = pv;
-                            ds[qq][kb][i] = pv * (dp[qq][kb][i] - (S)sd[r]);
+                            ds[qq][kb][i] = pv * (dp[qq][kb][i] - dr);
                         }
+                    }
                 Frag<P> pf[2], dsf[2];
 #pragma unroll
                 for (int kb = 0; kb < 2; kb++) {
@@ -682,6 +756,24 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
             __syncthreads();
         }
         // Lane holds dK/dV[key k0 + w*32 + kb*16 + crow(g, i)][d = db*16 + c].
+        if (a.parts == 2) {
+            S *PK = static_cast<S *>(a.pdk) + (part * a.batches + b) * a.tk * D;
+            S *PV = static_cast<S *>(a.pdv) + (part * a.batches + b) * a.tk * D;
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int64_t key = k0 + w * 32 + kb * 16 + crow<P>(g, i);
+                    if (key < a.tk) {
+#pragma unroll
+                        for (int db = 0; db < DB; db++) {
+                            PK[key * D + db * 16 + c] = dk[kb][db][i];
+                            PV[key * D + db * 16 + c] = dv[kb][db][i];
+                        }
+                    }
+                }
+            continue;
+        }
         In *DK = static_cast<In *>(a.dk) + b * a.tk * D;
         In *DV = static_cast<In *>(a.dv) + b * a.tk * D;
 #pragma unroll
@@ -703,7 +795,7 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
 // dQ for 128 queries per workgroup (32 per wave), visiting the key tiles
 // each may see.
 template <int P, int D>
-__global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
+__global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     typedef typename Input<P>::T In;
     typedef typename Acc<P>::type AccV;
     typedef typename Acc<P>::scalar S;
@@ -715,7 +807,7 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     __shared__ __attribute__((aligned(16))) E ks[PL * RPLANE];
     __shared__ __attribute__((aligned(16))) E ktr[PL * TPLANE];
     __shared__ __attribute__((aligned(16))) E vs[PL * RPLANE];
-    __shared__ int64_t red_lo[4], red_hi[4];
+    __shared__ int64_t red_lo[WAVES], red_hi[WAVES];
     __shared__ int slot;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
@@ -727,8 +819,10 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     for (;;) {
         const int item = take_item(a.ctr, &slot);
         if (item >= a.items) break;
-        const int64_t b = item % a.batches;
-        const int64_t qt = a.reverse ? nqt - 1 - item / a.batches : item / a.batches;
+        const int part = item % a.parts;
+        const int64_t blk = item / a.parts;
+        const int64_t b = blk % a.batches;
+        const int64_t qt = a.reverse ? nqt - 1 - blk / a.batches : blk / a.batches;
         const int64_t q0 = qt * BW;
         const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
         const In *DO = static_cast<const In *>(a.dout) + b * a.tq * D;
@@ -739,7 +833,7 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
         int64_t x[2];
         Span sp[2];
         Frag<P> qf[2][KC], df[2][KC];
-        S mrow[2], lrow[2], drow[2];
+        S mrow[2], drow[2];
 #pragma unroll
         for (int qb = 0; qb < 2; qb++) {
             x[qb] = q0 + w * 32 + qb * 16 + c;
@@ -755,23 +849,36 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
                 make_frag<P>(qf[qb][kc], t1);
                 make_frag<P>(df[qb][kc], t2);
             }
-            mrow[qb] = x[qb] < a.tq ? (S)a.row_max[b * a.tq + x[qb]] * (S)LOG2E : S(0);
-            lrow[qb] = x[qb] < a.tq ? (S)a.row_sum[b * a.tq + x[qb]] : S(1);
+            // m log2(e) + log2(l): P = exp2(scale s log2(e) - mrow).
+            mrow[qb] = x[qb] < a.tq ? (S)a.row_max[b * a.tq + x[qb]] * (S)LOG2E + (S)__log2f(a.row_sum[b * a.tq + x[qb]])
+                                    : S(0);
             drow[qb] = x[qb] < a.tq ? (S)a.delta[b * a.tq + x[qb]] : S(0);
         }
         int64_t first, last;
         group_range(sp, red_lo, red_hi, first, last);
+        int64_t jbeg = first < last ? first - first % BN : last;
+        if (a.parts == 2) {
+            const int64_t cut = jbeg + ((last - jbeg + 2 * BN - 1) / (2 * BN)) * BN;
+            if (part == 0) last = cut < last ? cut : last;
+            else jbeg = cut < last ? cut : last;
+        }
         AccV dq[2][DB];
 #pragma unroll
         for (int qb = 0; qb < 2; qb++)
 #pragma unroll
             for (int i = 0; i < DB; i++) dq[qb][i] = AccV{0, 0, 0, 0};
 
-        for (int64_t j0 = first < last ? first - first % BN : last; j0 < last; j0 += BN) {
+        for (int64_t j0 = jbeg; j0 < last; j0 += BN) {
             stage<E>(ks, RP, RPLANE, KR + j0 * D, D, rplane, BN, D, PL);
             stage<E>(ktr, TP, TPLANE, KT + j0, a.tkp, tplane, D, BN, PL);
             stage<E>(vs, RP, RPLANE, VR + j0 * D, D, rplane, BN, D, PL);
-            __syncthreads();
+            // Whether every query of the workgroup sees every real key of
+            // the tile (then no mask test per element).
+            const int64_t jvalid = j0 + BN < a.tk ? j0 + BN : a.tk;
+            bool full = true;
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++) full &= x[qb] >= a.tq || (sp[qb].lo <= j0 && sp[qb].hi >= jvalid);
+            const bool full_tile = __syncthreads_and(full);
             // The tile's keys in two 32-key halves: S^T = K Q^T and
             // dP^T = V dO^T, then dS^T, then dQ^T += K^T dS^T.
 #pragma unroll
@@ -806,9 +913,11 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
                     for (int kk = 0; kk < 2; kk++)
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
-                            const int64_t j = j0 + (2 * hc + kk) * 16 + crow<P>(g, i);
-                            const bool ok = j >= sp[qb].lo && j < sp[qb].hi;
-                            const S pv = ok ? fast_exp2(s[kk][qb][i] * scale2 - mrow[qb]) / lrow[qb] : S(0);
+                            S pv = fast_exp2(s[kk][qb][i] * scale2 - mrow[qb]);
+                            if (!full_tile) {
+                                const int64_t j = j0 + (2 * hc + kk) * 16 + crow<P>(g, i);
+                                if (!(j >= sp[qb].lo && j < sp[qb].hi)) pv = S(0);
+                            }
                             t[kk * 4 + i] = pv * (dp[kk][qb][i] - drow[qb]);
                         }
                     make_frag<P>(dsf[qb], t);
@@ -826,6 +935,18 @@ __global__ __launch_bounds__(256, 1) void attn_bwd_dq_kernel(BwdArgs a) {
             __syncthreads();
         }
         // Lane holds dQ^T[d = db*16 + crow(g, i)][query x].
+        if (a.parts == 2) {
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++) {
+                if (x[qb] >= a.tq) continue;
+                S *PQ = static_cast<S *>(a.pdq) + ((part * a.batches + b) * a.tq + x[qb]) * D;
+#pragma unroll
+                for (int db = 0; db < DB; db++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) PQ[db * 16 + crow<P>(g, i)] = dq[qb][db][i];
+            }
+            continue;
+        }
 #pragma unroll
         for (int qb = 0; qb < 2; qb++) {
             if (x[qb] >= a.tq) continue;
@@ -867,9 +988,16 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 bool fast_fwd_shape(const ef_attention_desc &d) {
     if (d.features != d.values) return false;
     // The kernel's 32-bit offsets into a plane of prepared operands.
-    if (2 * d.batches * (size_t)round_up((int64_t)d.keys, BW) * d.features >= (size_t(1) << 31)) return false;
+    if (2 * d.batches * (size_t)round_up((int64_t)d.keys, PAD) * d.features >= (size_t(1) << 31)) return false;
     if (d.precision == EF_PRECISION_FP64) return d.features == 32 || d.features == 64;
     return d.features == 32 || d.features == 64 || d.features == 128;
+}
+
+// Two parts per block where the mask makes blocks' work differ and there
+// are too few blocks to even it out over the CUs.
+int parts_for(const ef_attention_desc &d, int64_t blocks) {
+    const bool varying = d.allowed || d.causal;
+    return varying && blocks < 4 * (int64_t)compute_units() ? 2 : 1;
 }
 
 bool fast_bwd_shape(const ef_attention_desc &d) {
@@ -887,17 +1015,19 @@ size_t elem_bytes(uint32_t precision) {
 
 // Scratch layout, each region 256-byte aligned: the work counters, then
 // the forward's prepared K and V^T, or (from the same offset) the
-// backward's prepared tiles and each query's delta.
+// backward's prepared tiles and each query's delta; then the two parts'
+// partial results (forward sums, max and sum; backward dK, dV, dQ).
 struct Layout {
     size_t counters, kr, vt;
     size_t qr, qt, dor, dot, bkr, bkt, bvr, delta;
+    size_t po, pm, pl, pdk, pdv, pdq;
     size_t total;
 };
 
 Layout layout(const ef_attention_desc &d) {
     Layout L;
     const size_t e = elem_bytes(d.precision);
-    const size_t tqp = (size_t)round_up((int64_t)d.queries, BW), tkp = (size_t)round_up((int64_t)d.keys, BW);
+    const size_t tqp = (size_t)round_up((int64_t)d.queries, PAD), tkp = (size_t)round_up((int64_t)d.keys, PAD);
     const size_t qa = align256(d.batches * tqp * d.features * e), ka = align256(d.batches * tkp * d.features * e);
     size_t at = 0;
     L.counters = at; at += 256;
@@ -913,7 +1043,16 @@ Layout layout(const ef_attention_desc &d) {
     L.bkt = at; at += ka;
     L.bvr = at; at += ka;
     L.delta = at; at += align256(d.batches * d.queries * 4);
-    L.total = at > fwd_end ? at : fwd_end;
+    at = at > fwd_end ? at : fwd_end;
+    const size_t sb = d.precision == EF_PRECISION_FP64 ? 8 : 4;
+    const size_t rq = 2 * d.batches * d.queries, rk = 2 * d.batches * d.keys;
+    L.po = at; at += align256(rq * d.values * sb);
+    L.pm = at; at += align256(rq * sb);
+    L.pl = at; at += align256(rq * sb);
+    L.pdk = at; at += align256(rk * d.features * sb);
+    L.pdv = at; at += align256(rk * d.values * sb);
+    L.pdq = at; at += align256(rq * d.features * sb);
+    L.total = at;
     return L;
 }
 
@@ -924,7 +1063,7 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
     typedef typename Elem<P>::T E;
     const Layout L = layout(d);
     const int64_t B = (int64_t)d.batches, tq = (int64_t)d.queries, tk = (int64_t)d.keys;
-    const int64_t tkp = round_up(tk, BW);
+    const int64_t tkp = round_up(tk, PAD);
     E *kr = reinterpret_cast<E *>(scratch + L.kr);
     E *vt = reinterpret_cast<E *>(scratch + L.vt);
     dim3 pg((unsigned)(tkp / 64), (unsigned)B);
@@ -938,12 +1077,22 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
     a.mask = make_mask(d);
     a.tq = tq; a.tk = tk; a.tkp = tkp; a.batches = B;
     a.scale = d.scale;
-    a.items = (int)(B * ((tq + BW - 1) / BW));
+    const int64_t blocks = B * ((tq + BW - 1) / BW);
+    a.parts = parts_for(d, blocks);
+    a.pad = 0;
+    a.items = (int)(blocks * a.parts);
     a.reverse = d.causal ? 1 : 0;
+    a.po = scratch + L.po; a.pm = scratch + L.pm; a.pl = scratch + L.pl;
     const int cap = 2 * compute_units();
     const int grid = a.items < cap ? a.items : cap;
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
-    hipLaunchKernelGGL((attn_fwd_kernel<P, D>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((attn_fwd_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
+    if (a.parts == 2) {
+        typedef typename Acc<P>::scalar S;
+        const int64_t rows = B * tq;
+        hipLaunchKernelGGL((attn_merge_kernel<S, In>), dim3(blocks_for(rows * D, 256)), dim3(256), 0, s,
+                           (const S *)a.po, (const S *)a.pm, (const S *)a.pl, rows, D, (In *)out, row_max, row_sum);
+    }
     EF_HIP(hipGetLastError());
     return EF_OK;
 }
@@ -971,7 +1120,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
     typedef typename Elem<P>::T E;
     const Layout L = layout(d);
     const int64_t B = (int64_t)d.batches, tq = (int64_t)d.queries, tk = (int64_t)d.keys;
-    const int64_t tqp = round_up(tq, BW), tkp = round_up(tk, BW);
+    const int64_t tqp = round_up(tq, PAD), tkp = round_up(tk, PAD);
     BwdArgs a;
     a.q = q; a.k = k; a.v = v; a.dout = dout;
     a.qr = scratch + L.qr; a.qt = scratch + L.qt; a.dor = scratch + L.dor; a.dot = scratch + L.dot;
@@ -983,7 +1132,8 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
     a.tq = tq; a.tk = tk; a.tqp = tqp; a.tkp = tkp; a.batches = B;
     a.scale = d.scale;
     a.causal_only = (!d.allowed && d.causal && d.window <= 0) ? 1 : 0;
-    a.pad = 0;
+    a.pdk = scratch + L.pdk; a.pdv = scratch + L.pdv; a.pdq = scratch + L.pdq;
+    typedef typename Acc<P>::scalar S;
     if (stages & 1) {
         dim3 gq((unsigned)(tqp / 64), (unsigned)B), gk((unsigned)(tkp / 64), (unsigned)B);
         hipLaunchKernelGGL((prep_kernel<P, D>), gq, dim3(256), 0, s, (const In *)q, tq, tqp, B, (E *)a.qr, (E *)a.qt);
@@ -996,22 +1146,38 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         hipLaunchKernelGGL((attn_delta_kernel<In, D>), dim3((unsigned)((rows + 15) / 16)), dim3(256), 0, s,
                            (const In *)out, (const In *)dout, (float *)a.delta, rows);
     }
-    const int cap = compute_units();
+    const int cap = 2 * compute_units();
     if (stages & 2) {
         a.ctr = reinterpret_cast<Counter *>(scratch + L.counters);
-        a.items = (int)(B * ((tk + BW - 1) / BW));
+        const int64_t blocks = B * ((tk + BW - 1) / BW);
+        a.parts = parts_for(d, blocks);
+        a.items = (int)(blocks * a.parts);
         a.reverse = 0;   // key tiles in order: under a causal mask the first are the heaviest
         const int grid = a.items < cap ? a.items : cap;
         hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
-        hipLaunchKernelGGL((attn_bwd_dkdv_kernel<P, D>), dim3((unsigned)grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((attn_bwd_dkdv_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
+        if (a.parts == 2) {
+            const int64_t n = B * tk * D;
+            hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const S *)a.pdk, n,
+                               d.scale, (In *)dk);
+            hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const S *)a.pdv, n,
+                               1.0, (In *)dv);
+        }
     }
     if (stages & 4) {
         a.ctr = reinterpret_cast<Counter *>(scratch + L.counters) + 1;
-        a.items = (int)(B * ((tq + BW - 1) / BW));
+        const int64_t blocks = B * ((tq + BW - 1) / BW);
+        a.parts = parts_for(d, blocks);
+        a.items = (int)(blocks * a.parts);
         a.reverse = d.causal ? 1 : 0;   // under a causal mask the last query tiles are the heaviest
         const int grid = a.items < cap ? a.items : cap;
         hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
-        hipLaunchKernelGGL((attn_bwd_dq_kernel<P, D>), dim3((unsigned)grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((attn_bwd_dq_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
+        if (a.parts == 2) {
+            const int64_t n = B * tq * D;
+            hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const S *)a.pdq, n,
+                               d.scale, (In *)dq);
+        }
     }
     EF_HIP(hipGetLastError());
     return EF_OK;
