@@ -150,6 +150,39 @@ __device__ __forceinline__ void stage(E *lds, int lpitch, int lplane, const E *g
     }
 }
 
+// A tile of PLN planes x R rows x C elements in 16-byte chunks, spread
+// over the workgroup: thread t moves chunks t, t + THREADS, ...  All of a
+// thread's loads are issued before any is used, so a tile costs one
+// memory latency, and a caller can hold them in registers (prefetch)
+// while it computes on the previous tile.
+template <class E, int PLN, int R, int C> struct TileIO {
+    static constexpr int CE = 16 / sizeof(E);
+    static constexpr int PER_ROW = C / CE, PER_PLANE = R * PER_ROW, TOTAL = PLN * PER_PLANE;
+    static constexpr int N = (TOTAL + THREADS - 1) / THREADS;
+    __device__ static __forceinline__ void load(u32x4 (&buf)[N], const E *g, int64_t gstride, int64_t gplane) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int k = (int)threadIdx.x + THREADS * i;
+            if (TOTAL % THREADS == 0 || k < TOTAL) {
+                const int p = k / PER_PLANE, rem = k % PER_PLANE;
+                const int r = rem / PER_ROW, cc = (rem % PER_ROW) * CE;
+                buf[i] = *reinterpret_cast<const u32x4 *>(g + p * gplane + r * gstride + cc);
+            }
+        }
+    }
+    __device__ static __forceinline__ void store(E *lds, int lpitch, int lplane, const u32x4 (&buf)[N]) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int k = (int)threadIdx.x + THREADS * i;
+            if (TOTAL % THREADS == 0 || k < TOTAL) {
+                const int p = k / PER_PLANE, rem = k % PER_PLANE;
+                const int r = rem / PER_ROW, cc = (rem % PER_ROW) * CE;
+                *reinterpret_cast<u32x4 *>(lds + p * lplane + r * lpitch + cc) = buf[i];
+            }
+        }
+    }
+};
+
 // The union of the spans of a workgroup's queries (two per lane).
 __device__ __forceinline__ void group_range(const Span (&sp)[2], int64_t *red_lo, int64_t *red_hi, int64_t &first,
                                             int64_t &last) {
@@ -370,18 +403,27 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
 #pragma unroll
             for (int qb = 0; qb < 2; qb++) {
                 S mx = -INFINITY;
+                if (full_wg) {
 #pragma unroll
-                for (int kb = 0; kb < 4; kb++)
+                    for (int kb = 0; kb < 4; kb++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        S t = s[qb][kb][i] * scale2;
-                        if (!full_wg) {
-                            const int64_t j = j0 + kb * 16 + crow<P>(g, i);
-                            if (j < sp[qb].lo || j >= sp[qb].hi) t = -INFINITY;
+                        for (int i = 0; i < 4; i++) {
+                            const S t = s[qb][kb][i] * scale2;
+                            s[qb][kb][i] = t;
+                            mx = fmax(mx, t);
                         }
-                        s[qb][kb][i] = t;
-                        mx = fmax(mx, t);
-                    }
+                } else {
+                    const int64_t lo = sp[qb].lo, hi = sp[qb].hi;
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int64_t j = j0 + kb * 16 + crow<P>(g, i);
+                            const S t = j >= lo && j < hi ? s[qb][kb][i] * scale2 : S(-INFINITY);
+                            s[qb][kb][i] = t;
+                            mx = fmax(mx, t);
+                        }
+                }
                 mx = wave_max(mx);
                 const S mnew = fmax(mx, m[qb]);
                 S rs = 0;
@@ -662,26 +704,49 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
             if (part == 0) qt_end = cut;
             else qt_begin = cut;
         }
+        // The query tiles, the next one's operands and row statistics
+        // loaded into registers while this one is computed.
+        typedef TileIO<E, PL, BN, D> RowT;   // [query][feature]
+        typedef TileIO<E, PL, D, BN> ColT;   // [feature][query]
+        constexpr bool PF = 2 * (RowT::N + ColT::N) <= 16;
+        u32x4 pq[RowT::N], pqt[ColT::N], pd[RowT::N], pdt[ColT::N];
+        float pm = 0.f, pdl = 0.f;
+#define EF_FETCH(T_)                                                                         \
+    do {                                                                                     \
+        const int64_t q0_ = (T_) * BN;                                                       \
+        RowT::load(pq, QR + q0_ * D, D, rplane);                                             \
+        ColT::load(pqt, QT + q0_, a.tqp, tplane);                                            \
+        RowT::load(pd, DR + q0_ * D, D, rplane);                                             \
+        ColT::load(pdt, DT + q0_, a.tqp, tplane);                                            \
+        if (tid < BN) {                                                                      \
+            const int64_t xq_ = q0_ + tid;                                                   \
+            pm = xq_ < a.tq ? RM[xq_] * (float)LOG2E + __log2f(RS[xq_]) : 0.f;               \
+            pdl = xq_ < a.tq ? DL[xq_] : 0.f;                                                \
+        }                                                                                    \
+    } while (0)
+        if (PF && qt_begin < qt_end) EF_FETCH(qt_begin);
         for (int64_t qt = qt_begin; qt < qt_end; qt++) {
             const int64_t q0 = qt * BN;
+            if (!PF) EF_FETCH(qt);
+            RowT::store(qs, RP, RPLANE, pq);
+            ColT::store(qtr, TP, TPLANE, pqt);
+            RowT::store(dos, RP, RPLANE, pd);
+            ColT::store(dotr, TP, TPLANE, pdt);
             bool any = false, full = true;
             if (tid < BN) {
                 const int64_t xq = q0 + tid;
                 const Span sp = query_span(a.mask, xq);
                 slo[tid] = sp.lo;
                 shi[tid] = sp.hi;
-                sm[tid] = xq < a.tq ? RM[xq] * (float)LOG2E + __log2f(RS[xq]) : 0.f;
-                sd[tid] = xq < a.tq ? DL[xq] : 0.f;
+                sm[tid] = pm;
+                sd[tid] = pdl;
                 any = sp.lo < sp.hi && sp.lo < kvalid && sp.hi > k0;
                 full = xq >= a.tq || (sp.lo <= k0 && sp.hi >= kvalid);
             }
-            if (!__syncthreads_or(any)) continue;
+            const bool any_tile = __syncthreads_or(any);
             const bool full_tile = __syncthreads_and(full);
-            stage<E>(qs, RP, RPLANE, QR + q0 * D, D, rplane, BN, D, PL);
-            stage<E>(qtr, TP, TPLANE, QT + q0, a.tqp, tplane, D, BN, PL);
-            stage<E>(dos, RP, RPLANE, DR + q0 * D, D, rplane, BN, D, PL);
-            stage<E>(dotr, TP, TPLANE, DT + q0, a.tqp, tplane, D, BN, PL);
-            __syncthreads();
+            if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
+            if (any_tile) {
             // The tile's queries in two 32-query halves (hc): S = Q K^T and
             // dP = dO V^T for its two 16-query blocks, then P and dS, then
             // dV += P^T dO and dK += dS^T Q over the half.
@@ -712,20 +777,37 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                 // P = exp2(scale s log2(e) - m log2(e) - log2(l)), dS =
                 // P (dP - delta), 0 where masked (no test on a full tile).
                 S p[2][2][4], ds[2][2][4];
+                if (full_tile) {
 #pragma unroll
-                for (int qq = 0; qq < 2; qq++)
+                    for (int qq = 0; qq < 2; qq++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
-                        const S mr = (S)sm[r], dr = (S)sd[r];
+                        for (int i = 0; i < 4; i++) {
+                            const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
+                            const S mr = (S)sm[r], dr = (S)sd[r];
 #pragma unroll
-                        for (int kb = 0; kb < 2; kb++) {
-                            S pv = fast_exp2(s[qq][kb][i] * scale2 - mr);
-                            if (!full_tile && !(kj[kb] >= slo[r] && kj[kb] < shi[r])) pv = S(0);
-                            p[qq][kb][i] = pv;
-                            ds[qq][kb][i] = pv * (dp[qq][kb][i] - dr);
+                            for (int kb = 0; kb < 2; kb++) {
+                                const S pv = fast_exp2(s[qq][kb][i] * scale2 - mr);
+                                p[qq][kb][i] = pv;
+                                ds[qq][kb][i] = pv * (dp[qq][kb][i] - dr);
+                            }
                         }
-                    }
+                } else {
+#pragma unroll
+                    for (int qq = 0; qq < 2; qq++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
+                            const S mr = (S)sm[r], dr = (S)sd[r];
+                            const int64_t lo = slo[r], hi = shi[r];
+#pragma unroll
+                            for (int kb = 0; kb < 2; kb++) {
+                                const S e = fast_exp2(s[qq][kb][i] * scale2 - mr);
+                                const S pv = kj[kb] >= lo && kj[kb] < hi ? e : S(0);
+                                p[qq][kb][i] = pv;
+                                ds[qq][kb][i] = pv * (dp[qq][kb][i] - dr);
+                            }
+                        }
+                }
                 Frag<P> pf[2], dsf[2];
 #pragma unroll
                 for (int kb = 0; kb < 2; kb++) {
@@ -753,8 +835,10 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                         }
                 }
             }
+            }
             __syncthreads();
         }
+#undef EF_FETCH
         // Lane holds dK/dV[key k0 + w*32 + kb*16 + crow(g, i)][d = db*16 + c].
         if (a.parts == 2) {
             S *PK = static_cast<S *>(a.pdk) + (part * a.batches + b) * a.tk * D;
@@ -868,10 +952,24 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
 #pragma unroll
             for (int i = 0; i < DB; i++) dq[qb][i] = AccV{0, 0, 0, 0};
 
+        // The key tiles, the next one's loaded into registers while this one
+        // is computed.
+        typedef TileIO<E, PL, BN, D> RowT;   // [key][feature]
+        typedef TileIO<E, PL, D, BN> ColT;   // [feature][key]
+        constexpr bool PF = 2 * RowT::N + ColT::N <= 16;
+        u32x4 pk[RowT::N], pkt[ColT::N], pv_[RowT::N];
+#define EF_FETCH(J_)                                                                         \
+    do {                                                                                     \
+        RowT::load(pk, KR + (J_) * D, D, rplane);                                            \
+        ColT::load(pkt, KT + (J_), a.tkp, tplane);                                           \
+        RowT::load(pv_, VR + (J_) * D, D, rplane);                                           \
+    } while (0)
+        if (PF && jbeg < last) EF_FETCH(jbeg);
         for (int64_t j0 = jbeg; j0 < last; j0 += BN) {
-            stage<E>(ks, RP, RPLANE, KR + j0 * D, D, rplane, BN, D, PL);
-            stage<E>(ktr, TP, TPLANE, KT + j0, a.tkp, tplane, D, BN, PL);
-            stage<E>(vs, RP, RPLANE, VR + j0 * D, D, rplane, BN, D, PL);
+            if (!PF) EF_FETCH(j0);
+            RowT::store(ks, RP, RPLANE, pk);
+            ColT::store(ktr, TP, TPLANE, pkt);
+            RowT::store(vs, RP, RPLANE, pv_);
             // Whether every query of the workgroup sees every real key of
             // the tile (then no mask test per element).
             const int64_t jvalid = j0 + BN < a.tk ? j0 + BN : a.tk;
@@ -879,6 +977,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
 #pragma unroll
             for (int qb = 0; qb < 2; qb++) full &= x[qb] >= a.tq || (sp[qb].lo <= j0 && sp[qb].hi >= jvalid);
             const bool full_tile = __syncthreads_and(full);
+            if (PF && j0 + BN < last) EF_FETCH(j0 + BN);
             // The tile's keys in two 32-key halves: S^T = K Q^T and
             // dP^T = V dO^T, then dS^T, then dQ^T += K^T dS^T.
 #pragma unroll
@@ -909,17 +1008,25 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
 #pragma unroll
                 for (int qb = 0; qb < 2; qb++) {
                     S t[8];
+                    if (full_tile) {
 #pragma unroll
-                    for (int kk = 0; kk < 2; kk++)
+                        for (int kk = 0; kk < 2; kk++)
 #pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            S pv = fast_exp2(s[kk][qb][i] * scale2 - mrow[qb]);
-                            if (!full_tile) {
-                                const int64_t j = j0 + (2 * hc + kk) * 16 + crow<P>(g, i);
-                                if (!(j >= sp[qb].lo && j < sp[qb].hi)) pv = S(0);
+                            for (int i = 0; i < 4; i++) {
+                                const S pv = fast_exp2(s[kk][qb][i] * scale2 - mrow[qb]);
+                                t[kk * 4 + i] = pv * (dp[kk][qb][i] - drow[qb]);
                             }
-                            t[kk * 4 + i] = pv * (dp[kk][qb][i] - drow[qb]);
-                        }
+                    } else {
+#pragma unroll
+                        for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                const int64_t j = j0 + (2 * hc + kk) * 16 + crow<P>(g, i);
+                                const S e = fast_exp2(s[kk][qb][i] * scale2 - mrow[qb]);
+                                const S pv = j >= sp[qb].lo && j < sp[qb].hi ? e : S(0);
+                                t[kk * 4 + i] = pv * (dp[kk][qb][i] - drow[qb]);
+                            }
+                    }
                     make_frag<P>(dsf[qb], t);
                 }
                 Frag<P> kt[DB];
@@ -934,6 +1041,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
             }
             __syncthreads();
         }
+#undef EF_FETCH
         // Lane holds dQ^T[d = db*16 + crow(g, i)][query x].
         if (a.parts == 2) {
 #pragma unroll
