@@ -157,7 +157,8 @@ def main():
         cpu = cpu_baseline()
     secondary = None
     if not args.no_secondary:
-        secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev)}
+        secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev),
+                     "batched_folds": batched_folds_line(dev)}
 
     line = {
         "metric": "fold GFLOP/s + attention fwd+bwd ms: mixed vs plain FP32, 1 MI355X",
@@ -247,6 +248,34 @@ def dense_fold_line(dev, with_cpu: bool):
         dt = (time.perf_counter() - t0) / reps
         line["cpu_port_ms"] = round(dt * 1e3, 3)
         line["cpu_port_cores"] = 1
+    return line
+
+
+def batched_folds_line(dev):
+    """64 independent dense Float32 folds, M, N and K each log-uniform in
+    [32, 2048] (seed 9), in one persistent graph-replayed launch; the
+    autotuner picks the tile shape.  FLOP = sum of 2 M N K."""
+    import torch
+    from einfold_amd import data, device
+    shapes = data.log_uniform_shapes(64, 32, 2048, seed=9)
+    g = torch.Generator().manual_seed(9)
+    probs = [(torch.randn(m, k, generator=g).to(dev), torch.randn(k, n, generator=g).to(dev),
+              torch.empty(m, n, device=dev)) for m, n, k in shapes]
+    flop = float(sum(2.0 * m * n * k for m, n, k in shapes))
+    line = {"workload": "64 uneven dense FP32 folds, M,N,K log-uniform in [32, 2048]", "gflop": round(flop / 1e9, 2)}
+    for precision in ("bf16x3", "fp32"):
+        plan = device.GroupedPlan(probs, precision)
+        ms = _events(plan.run, 10)
+        a, b, c = probs[0]
+        err = float(torch.linalg.vector_norm(c.double() - a.double() @ b.double()) /
+                    torch.linalg.vector_norm(a.double() @ b.double()))
+        tf = flop / (ms * 1e-3) / 1e12
+        peak = PEAK_BF16 if precision == "bf16x3" else PEAK_FP32
+        line[precision] = {"ms": round(ms, 4), "tflops": round(tf, 1), "tile": plan.tile,
+                           "frac_of_peak": round(tf / (peak / 1e12), 4),
+                           "issued_frac": round((3 if precision == "bf16x3" else 1) * tf / (peak / 1e12), 4),
+                           "rel_err_fold0": err}
+        plan.close()
     return line
 
 

@@ -28,6 +28,7 @@ EXPORTS = [
     "ef_rank_scratch_bytes", "ef_rank_u64",
     "ef_attention_scratch_bytes", "ef_attention_fwd", "ef_attention_bwd",
     "ef_attention_bwd_prepare", "ef_attention_bwd_kv", "ef_attention_bwd_q",
+    "ef_grouped_plan_create", "ef_grouped_plan_run", "ef_grouped_plan_tile", "ef_grouped_plan_destroy",
 ]
 
 
@@ -59,6 +60,11 @@ class CAttentionShape(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_size_t), ("queries", ctypes.c_size_t),
                 ("features", ctypes.c_size_t), ("keys", ctypes.c_size_t),
                 ("values", ctypes.c_size_t)]
+
+
+class CGemmProblem(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("m", ctypes.c_size_t), ("n", ctypes.c_size_t), ("k", ctypes.c_size_t)]
 
 
 class CRelation(ctypes.Structure):
@@ -162,6 +168,15 @@ def lib():
         L.ef_attention_bwd_kv.restype = st
         L.ef_attention_bwd_q.argtypes = [D, P, P, P, P, P, P, P]
         L.ef_attention_bwd_q.restype = st
+        L.ef_grouped_plan_create.argtypes = [ctypes.POINTER(CGemmProblem), sz, ctypes.c_uint32, ctypes.c_int, P,
+                                             ctypes.POINTER(P)]
+        L.ef_grouped_plan_create.restype = st
+        L.ef_grouped_plan_run.argtypes = [P, P]
+        L.ef_grouped_plan_run.restype = st
+        L.ef_grouped_plan_tile.argtypes = [P]
+        L.ef_grouped_plan_tile.restype = ctypes.c_int
+        L.ef_grouped_plan_destroy.argtypes = [P]
+        L.ef_grouped_plan_destroy.restype = None
         _lib = L
     return _lib
 

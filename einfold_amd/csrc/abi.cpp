@@ -80,6 +80,30 @@ ef_status ef_rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t 
     return rank_u64(words, n, rank, first, n_ranks, scratch, as_stream(stream));
 }
 
+ef_status ef_grouped_plan_create(const ef_gemm_problem *problems, size_t count, uint32_t precision, int tile,
+                                 void *stream, ef_grouped_plan **plan) {
+    clear_error();
+    if (!plan || (count && !problems)) {
+        set_error("problems and a plan to write are required");
+        return EF_INVALID;
+    }
+    GroupedPlan *p = nullptr;
+    const ef_status st = grouped_plan_create(problems, count, precision, tile, stream, &p);
+    *plan = reinterpret_cast<ef_grouped_plan *>(p);
+    return st;
+}
+
+ef_status ef_grouped_plan_run(ef_grouped_plan *plan, void *stream) {
+    clear_error();
+    return grouped_plan_run(reinterpret_cast<GroupedPlan *>(plan), stream);
+}
+
+int ef_grouped_plan_tile(const ef_grouped_plan *plan) {
+    return grouped_plan_tile(reinterpret_cast<const GroupedPlan *>(plan));
+}
+
+void ef_grouped_plan_destroy(ef_grouped_plan *plan) { grouped_plan_destroy(reinterpret_cast<GroupedPlan *>(plan)); }
+
 size_t ef_attention_scratch_bytes(const ef_attention_desc *desc) {
     return desc ? attention_scratch_bytes(*desc) : 0;
 }

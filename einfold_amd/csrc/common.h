@@ -82,4 +82,11 @@ ef_status attention_bwd(const ef_attention_desc &d, const void *q, const void *k
                         const float *row_sum, void *dq, void *dk, void *dv, void *scratch,
                         int stages, hipStream_t s);   // stages: 1 prepare, 2 dK/dV, 4 dQ
 
+struct GroupedPlan;
+ef_status grouped_plan_create(const ef_gemm_problem *problems, size_t count, uint32_t precision, int tile,
+                              void *stream, GroupedPlan **out);
+ef_status grouped_plan_run(GroupedPlan *plan, void *stream);
+int grouped_plan_tile(const GroupedPlan *plan);
+void grouped_plan_destroy(GroupedPlan *plan);
+
 }  // namespace ef

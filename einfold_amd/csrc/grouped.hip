@@ -1,0 +1,451 @@
+// Many uneven dense products in one persistent launch: einfold's batched
+// dense folds (dense.rs:706-770 with Float32 factors, products C = A B per
+// fold) as a grouped GEMM on MFMA.
+//
+// A prep launch writes every problem's operands once in the form the
+// fragments read: A as [plane][m][k] and B transposed as [plane][n][k]
+// (k contiguous in both), padded with zeros to whole tiles, split into BF16
+// hi and lo planes for BF16x3.  The GEMM launch is persistent: workgroups
+// take (problem, tile row, tile column) items from a counter, largest K
+// first, and compute each output tile whole, k in order, so the bits of a
+// tile never depend on which workgroup took it or on the other problems.
+// Tiles stream through double-buffered LDS with the next k step held in
+// registers while the current one is multiplied.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+#include "mfma.h"
+
+namespace ef {
+
+namespace {
+
+constexpr int KS = 32;          // k per step
+constexpr int PADT = 128;       // rows and columns padded to this
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+struct GemmDesc {
+    const float *a, *b;
+    float *c;
+    int64_t m, n, k;
+    int64_t mp, np, kp;          // padded
+    int64_t a_off, b_off;        // element offsets of the prepared planes in the operand buffer
+};
+
+struct PrepTask {
+    int problem, which;          // which 0: A, 1: B
+    int bi, bj;                  // 64 x 64 block of the destination (rows, k)
+};
+
+struct Item {
+    int problem, tm, tn, pad;
+};
+
+struct Counter { unsigned next, pad[3]; };
+
+// Writes one 64 x 64 block of a prepared operand: destination rows r (of m
+// or n) and columns kk (of k).  A is read as [m][k]; B as [k][n] and
+// transposed through LDS.
+template <int P>
+__global__ __launch_bounds__(256) void grouped_prep_kernel(const GemmDesc *descs, const PrepTask *tasks,
+                                                           typename Elem<P>::T *ops) {
+    typedef typename Elem<P>::T E;
+    __shared__ float t[64][65];
+    const PrepTask task = tasks[blockIdx.x];
+    const GemmDesc d = descs[task.problem];
+    const int64_t r0 = (int64_t)task.bi * 64, k0 = (int64_t)task.bj * 64;
+    const int tid = threadIdx.x;
+    if (task.which == 0) {
+        const int64_t plane = d.mp * d.kp;
+        E *dst = ops + d.a_off;
+        for (int i = tid; i < 64 * 64; i += 256) {
+            const int r = i / 64, kk = i % 64;
+            const int64_t row = r0 + r, col = k0 + kk;
+            const float v = (row < d.m && col < d.k) ? d.a[row * d.k + col] : 0.f;
+            if (row < d.mp && col < d.kp) {
+                if constexpr (P == PREC_BF16X3) {
+                    __bf16 h, l;
+                    split_bf16(v, h, l);
+                    dst[row * d.kp + col] = h;
+                    dst[plane + row * d.kp + col] = l;
+                } else {
+                    dst[row * d.kp + col] = (E)v;
+                }
+            }
+        }
+    } else {
+        const int64_t plane = d.np * d.kp;
+        E *dst = ops + d.b_off;
+        // Read B[k0 + kk][r0 + r] (coalesced along n), write [r][kk].
+        for (int i = tid; i < 64 * 64; i += 256) {
+            const int kk = i / 64, r = i % 64;
+            const int64_t kx = k0 + kk, col = r0 + r;
+            t[kk][r] = (kx < d.k && col < d.n) ? d.b[kx * d.n + col] : 0.f;
+        }
+        __syncthreads();
+        for (int i = tid; i < 64 * 64; i += 256) {
+            const int r = i / 64, kk = i % 64;
+            const int64_t row = r0 + r, col = k0 + kk;
+            if (row < d.np && col < d.kp) {
+                const float v = t[kk][r];
+                if constexpr (P == PREC_BF16X3) {
+                    __bf16 h, l;
+                    split_bf16(v, h, l);
+                    dst[row * d.kp + col] = h;
+                    dst[plane + row * d.kp + col] = l;
+                } else {
+                    dst[row * d.kp + col] = (E)v;
+                }
+            }
+        }
+    }
+}
+
+__global__ void grouped_zero_kernel(Counter *c) {
+    if (threadIdx.x == 0) c->next = 0;
+}
+
+// One workgroup of 4 waves computes a BM x BN output tile; wave (wr, wc)
+// owns a (BM/2) x (BN/2) quarter as 16 x 16 MFMA blocks.
+template <int P, int BM, int BN>
+__global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, int n_items,
+                                                              const typename Elem<P>::T *ops, Counter *ctr) {
+    typedef typename Elem<P>::T E;
+    typedef typename Acc<P>::type AccV;
+    constexpr int PL = Elem<P>::planes;
+    constexpr int CE = 16 / sizeof(E);
+    constexpr int PITCH = KS + CE;   // one chunk of padding per row
+    constexpr int APLANE = BM * PITCH, BPLANE = BN * PITCH;
+    constexpr int BUF = PL * (APLANE + BPLANE);
+    constexpr int ACH = PL * BM * (KS / CE), BCH = PL * BN * (KS / CE);
+    constexpr int AN = ACH / 256, BNN = BCH / 256, N = AN + BNN;
+    constexpr int WM = BM / 2, WN = BN / 2, IM = WM / 16, JN = WN / 16;
+    constexpr int STEPS = KS / 32;   // 32-wide fragment chunks per k step
+    __shared__ __attribute__((aligned(16))) E lds[2 * BUF];
+    __shared__ int slot;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
+    const int wr = w >> 1, wc = w & 1;
+    // This thread's chunks: (plane, row, col) of the A and B tiles.
+    int crow_[N], ccol[N], cpl[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int kq = tid + 256 * (i < AN ? i : i - AN);
+        const int per_row = KS / CE;
+        const int rows = i < AN ? BM : BN;
+        cpl[i] = kq / (rows * per_row);
+        crow_[i] = (kq % (rows * per_row)) / per_row;
+        ccol[i] = (kq % per_row) * CE;
+    }
+    for (;;) {
+        if (tid == 0) slot = (int)atomicAdd(&ctr->next, 1u);
+        __syncthreads();
+        const int it = slot;
+        __syncthreads();
+        if (it >= n_items) break;
+        const Item item = items[it];
+        const GemmDesc d = descs[item.problem];
+        const int64_t m0 = (int64_t)item.tm * BM, n0 = (int64_t)item.tn * BN;
+        const E *A = ops + d.a_off + m0 * d.kp;
+        const E *B = ops + d.b_off + n0 * d.kp;
+        const int64_t aplane = d.mp * d.kp, bplane = d.np * d.kp;
+        AccV acc[IM][JN];
+#pragma unroll
+        for (int i = 0; i < IM; i++)
+#pragma unroll
+            for (int j = 0; j < JN; j++) acc[i][j] = AccV{0, 0, 0, 0};
+        u32x4 pre[N];
+#define EF_GLOAD(K0)                                                                                  \
+    _Pragma("unroll") for (int i_ = 0; i_ < N; i_++) {                                               \
+        const E *src_ = i_ < AN ? A + cpl[i_] * aplane : B + cpl[i_] * bplane;                        \
+        pre[i_] = *reinterpret_cast<const u32x4 *>(src_ + (int64_t)crow_[i_] * d.kp + (K0) + ccol[i_]); \
+    }
+#define EF_LSTORE(BUFP)                                                                               \
+    _Pragma("unroll") for (int i_ = 0; i_ < N; i_++) {                                               \
+        E *dst_ = (BUFP) + (i_ < AN ? cpl[i_] * APLANE : PL * APLANE + cpl[i_] * BPLANE);             \
+        *reinterpret_cast<u32x4 *>(dst_ + crow_[i_] * PITCH + ccol[i_]) = pre[i_];                    \
+    }
+        const int64_t nk = d.kp / KS;
+        EF_GLOAD(0);
+        EF_LSTORE(lds);
+        __syncthreads();
+        for (int64_t ks = 0; ks < nk; ks++) {
+            const int cur = (int)(ks & 1);
+            if (ks + 1 < nk) EF_GLOAD((ks + 1) * KS);
+            const E *as = lds + cur * BUF;
+            const E *bs = as + PL * APLANE;
+#pragma unroll
+            for (int st = 0; st < STEPS; st++) {
+                Frag<P> af[IM], bf[JN];
+#pragma unroll
+                for (int i = 0; i < IM; i++) tile_load8<P>(af[i], as, APLANE, (wr * WM + i * 16 + c) * PITCH + st * 32 + g * 8);
+#pragma unroll
+                for (int j = 0; j < JN; j++) tile_load8<P>(bf[j], bs, BPLANE, (wc * WN + j * 16 + c) * PITCH + st * 32 + g * 8);
+#pragma unroll
+                for (int sp = 0; sp < Parts<P>::n; sp++)
+#pragma unroll
+                    for (int i = 0; i < IM; i++)
+#pragma unroll
+                        for (int j = 0; j < JN; j++) mma_part<P>(acc[i][j], af[i], bf[j], sp);
+            }
+            if (ks + 1 < nk) EF_LSTORE(lds + (cur ^ 1) * BUF);
+            __syncthreads();
+        }
+#undef EF_GLOAD
+#undef EF_LSTORE
+        // Lane holds C[m0 + wr*WM + i*16 + crow(g, r)][n0 + wc*WN + j*16 + c].
+#pragma unroll
+        for (int i = 0; i < IM; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = m0 + wr * WM + i * 16 + crow<P>(g, r);
+                if (row >= d.m) continue;
+#pragma unroll
+                for (int j = 0; j < JN; j++) {
+                    const int64_t col = n0 + wc * WN + j * 16 + c;
+                    if (col < d.n) d.c[row * d.n + col] = acc[i][j][r];
+                }
+            }
+    }
+}
+
+size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int tile_rows(int config) { return config == 0 ? 128 : 64; }
+
+}  // namespace
+
+// A prepared set of products: descriptors, prep tasks and work items on the
+// device, the prepared operands, and an instantiated graph of
+// prep + counter reset + GEMM on the plan's stream.
+struct GroupedPlan {
+    std::vector<ef_gemm_problem> problems;
+    uint32_t precision = EF_PRECISION_BF16X3;
+    int config = 0;                   // tile shape: 0 = 128 x 128, 1 = 64 x 64
+    DeviceBuffer mem;
+    GemmDesc *descs = nullptr;
+    PrepTask *tasks = nullptr;
+    Item *items = nullptr;
+    Counter *ctr = nullptr;
+    void *ops = nullptr;
+    int n_tasks = 0, n_items = 0, grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GroupedPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+int cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
+                                               hipSuccess && n > 0)
+        return n;
+    return 256;
+}
+
+ef_status build(GroupedPlan &p, hipStream_t s) {
+    const int T = tile_rows(p.config);
+    const size_t esz = p.precision == EF_PRECISION_FP32 ? 4 : 2;
+    const size_t planes = p.precision == EF_PRECISION_BF16X3 ? 2 : 1;
+    std::vector<GemmDesc> descs;
+    std::vector<PrepTask> tasks;
+    std::vector<std::pair<int64_t, Item>> items;
+    int64_t off = 0;
+    for (size_t i = 0; i < p.problems.size(); i++) {
+        const ef_gemm_problem &q = p.problems[i];
+        GemmDesc d;
+        d.a = q.a; d.b = q.b; d.c = q.c;
+        d.m = (int64_t)q.m; d.n = (int64_t)q.n; d.k = (int64_t)q.k;
+        d.mp = (d.m + PADT - 1) / PADT * PADT;
+        d.np = (d.n + PADT - 1) / PADT * PADT;
+        d.kp = (d.k + 63) / 64 * 64;
+        d.a_off = off; off += (int64_t)planes * d.mp * d.kp;
+        d.b_off = off; off += (int64_t)planes * d.np * d.kp;
+        descs.push_back(d);
+        if (d.m == 0 || d.n == 0) continue;
+        for (int bi = 0; bi < d.mp / 64; bi++)
+            for (int bj = 0; bj < d.kp / 64; bj++) tasks.push_back({(int)i, 0, bi, bj});
+        for (int bi = 0; bi < d.np / 64; bi++)
+            for (int bj = 0; bj < d.kp / 64; bj++) tasks.push_back({(int)i, 1, bi, bj});
+        for (int tm = 0; tm < (d.m + T - 1) / T; tm++)
+            for (int tn = 0; tn < (d.n + T - 1) / T; tn++) items.push_back({d.kp, Item{(int)i, tm, tn, 0}});
+    }
+    // Largest k first; ties in problem and tile order (deterministic list).
+    std::stable_sort(items.begin(), items.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    p.n_tasks = (int)tasks.size();
+    p.n_items = (int)items.size();
+    const size_t bytes = up256(descs.size() * sizeof(GemmDesc)) + up256(tasks.size() * sizeof(PrepTask)) +
+                         up256(items.size() * sizeof(Item)) + 256 + up256((size_t)off * esz);
+    EF_TRY(p.mem.alloc(bytes));
+    char *at = static_cast<char *>(p.mem.ptr);
+    p.descs = reinterpret_cast<GemmDesc *>(at); at += up256(descs.size() * sizeof(GemmDesc));
+    p.tasks = reinterpret_cast<PrepTask *>(at); at += up256(tasks.size() * sizeof(PrepTask));
+    p.items = reinterpret_cast<Item *>(at); at += up256(items.size() * sizeof(Item));
+    p.ctr = reinterpret_cast<Counter *>(at); at += 256;
+    p.ops = at;
+    std::vector<Item> flat;
+    for (auto &x : items) flat.push_back(x.second);
+    if (!descs.empty()) EF_HIP(hipMemcpyAsync(p.descs, descs.data(), descs.size() * sizeof(GemmDesc), hipMemcpyHostToDevice, s));
+    if (!tasks.empty()) EF_HIP(hipMemcpyAsync(p.tasks, tasks.data(), tasks.size() * sizeof(PrepTask), hipMemcpyHostToDevice, s));
+    if (!flat.empty()) EF_HIP(hipMemcpyAsync(p.items, flat.data(), flat.size() * sizeof(Item), hipMemcpyHostToDevice, s));
+    EF_HIP(hipStreamSynchronize(s));
+    // Up to four workgroups per CU where registers and LDS allow; the rest
+    // start when a slot frees and find the work list empty.
+    p.grid = std::min(p.n_items, 4 * cus());
+    return EF_OK;
+}
+
+template <int P>
+ef_status launch_steps(const GroupedPlan &p, hipStream_t s) {
+    typedef typename Elem<P>::T E;
+    if (p.n_items == 0) return EF_OK;
+    hipLaunchKernelGGL(grouped_prep_kernel<P>, dim3((unsigned)p.n_tasks), dim3(256), 0, s, p.descs, p.tasks, (E *)p.ops);
+    hipLaunchKernelGGL(grouped_zero_kernel, dim3(1), dim3(64), 0, s, p.ctr);
+    if (p.config == 0)
+        hipLaunchKernelGGL((grouped_gemm_kernel<P, 128, 128>), dim3((unsigned)p.grid), dim3(256), 0, s, p.descs,
+                           p.items, p.n_items, (const E *)p.ops, p.ctr);
+    else
+        hipLaunchKernelGGL((grouped_gemm_kernel<P, 64, 64>), dim3((unsigned)p.grid), dim3(256), 0, s, p.descs,
+                           p.items, p.n_items, (const E *)p.ops, p.ctr);
+    EF_HIP(hipGetLastError());
+    return EF_OK;
+}
+
+ef_status launch_all(const GroupedPlan &p, hipStream_t s) {
+    switch (p.precision) {
+    case EF_PRECISION_FP32: return launch_steps<PREC_F32>(p, s);
+    case EF_PRECISION_BF16X3: return launch_steps<PREC_BF16X3>(p, s);
+    case EF_PRECISION_BF16: return launch_steps<PREC_BF16>(p, s);
+    default: set_error("grouped products take FP32, BF16X3 or BF16"); return EF_INVALID;
+    }
+}
+
+// The autotuner's memory: the fastest tile shape per (precision, shape
+// signature), measured once per process.
+std::mutex tune_mu;
+std::map<std::vector<int64_t>, int> tuned;
+
+std::vector<int64_t> signature(const GroupedPlan &p) {
+    // Shapes bucketed to powers of two, sorted: similar sets share a choice.
+    std::vector<int64_t> sig = {(int64_t)p.precision};
+    std::vector<int64_t> shapes;
+    auto bucket = [](size_t v) { int64_t b = 0; while (((size_t)1 << b) < v) b++; return b; };
+    for (auto &q : p.problems) shapes.push_back(bucket(q.m) << 16 | bucket(q.n) << 8 | bucket(q.k));
+    std::sort(shapes.begin(), shapes.end());
+    sig.insert(sig.end(), shapes.begin(), shapes.end());
+    return sig;
+}
+
+ef_status time_config(GroupedPlan &p, hipStream_t s, float &ms) {
+    EF_TRY(build(p, s));
+    hipEvent_t a, b;
+    EF_HIP(hipEventCreate(&a));
+    EF_HIP(hipEventCreate(&b));
+    EF_TRY(launch_all(p, s));   // warm
+    EF_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < 3; r++) EF_TRY(launch_all(p, s));
+    EF_HIP(hipEventRecord(b, s));
+    EF_HIP(hipEventSynchronize(b));
+    EF_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return EF_OK;
+}
+
+}  // namespace
+
+ef_status grouped_plan_create(const ef_gemm_problem *problems, size_t count, uint32_t precision, int tile,
+                              void *stream, GroupedPlan **out) {
+    *out = nullptr;
+    hipStream_t s = as_stream(stream);
+    if (precision != EF_PRECISION_FP32 && precision != EF_PRECISION_BF16X3 && precision != EF_PRECISION_BF16) {
+        set_error("grouped products take FP32, BF16X3 or BF16");
+        return EF_INVALID;
+    }
+    auto plan = new GroupedPlan();
+    plan->problems.assign(problems, problems + count);
+    plan->precision = precision;
+    ef_status st = EF_OK;
+    if (tile == 128 || tile == 64) {
+        plan->config = tile == 128 ? 0 : 1;
+    } else {
+        const auto sig = signature(*plan);
+        int known = -1;
+        {
+            std::lock_guard<std::mutex> lock(tune_mu);
+            auto it = tuned.find(sig);
+            if (it != tuned.end()) known = it->second;
+        }
+        if (known >= 0) {
+            plan->config = known;
+        } else {
+            float best = 1e30f;
+            int best_cfg = 0;
+            for (int cfg = 0; cfg < 2 && st == EF_OK; cfg++) {
+                plan->config = cfg;
+                float ms = 0;
+                st = time_config(*plan, s, ms);
+                if (st == EF_OK && ms < best) { best = ms; best_cfg = cfg; }
+            }
+            plan->config = best_cfg;
+            std::lock_guard<std::mutex> lock(tune_mu);
+            tuned[sig] = best_cfg;
+        }
+    }
+    if (st == EF_OK) st = build(*plan, s);
+    // One graph of the three launches, replayed by each run.
+    if (st == EF_OK && plan->n_items > 0) {
+        hipStream_t cap;
+        if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+            st = hip_status(hipGetLastError(), "hipStreamCreate");
+        } else {
+            hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                st = launch_all(*plan, cap);
+                hipGraph_t g = nullptr;
+                hipError_t e2 = hipStreamEndCapture(cap, &g);
+                if (st == EF_OK && e2 == hipSuccess) {
+                    plan->graph = g;
+                    e2 = hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0);
+                }
+                if (st == EF_OK && e2 != hipSuccess) st = hip_status(e2, "graph capture");
+            } else {
+                st = hip_status(e, "hipStreamBeginCapture");
+            }
+            (void)hipStreamDestroy(cap);
+        }
+    }
+    if (st != EF_OK) {
+        delete plan;
+        return st;
+    }
+    *out = plan;
+    return EF_OK;
+}
+
+ef_status grouped_plan_run(GroupedPlan *plan, void *stream) {
+    if (!plan) {
+        set_error("no plan");
+        return EF_INVALID;
+    }
+    if (plan->n_items == 0) return EF_OK;
+    EF_HIP(hipGraphLaunch(plan->exec, as_stream(stream)));
+    return EF_OK;
+}
+
+int grouped_plan_tile(const GroupedPlan *plan) { return plan ? tile_rows(plan->config) : 0; }
+
+void grouped_plan_destroy(GroupedPlan *plan) { delete plan; }
+
+}  // namespace ef

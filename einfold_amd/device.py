@@ -156,3 +156,45 @@ def attention_bwd(q, k, v, out, dout, row_max, row_sum, scale: float, causal: bo
                                  _ptr(dout), _ptr(row_max), _ptr(row_sum), _ptr(dq), _ptr(dk),
                                  _ptr(dvv), _ptr(scratch), _stream()))
     return dq, dk, dvv
+
+
+class GroupedPlan:
+    """Many uneven products c = a @ b (float32 tensors on the GPU, row-major)
+    in one persistent launch replayed from a HIP graph.  ``tile`` 64 or 128
+    fixes the tile shape; 0 lets the autotuner choose.  The plan holds the
+    tensors' addresses: keep them alive, and build a new plan if they move."""
+
+    def __init__(self, problems, precision: str = "bf16x3", tile: int = 0):
+        self._keep = []
+        arr = (_lib.CGemmProblem * max(1, len(problems)))()
+        for i, (a, b, c) in enumerate(problems):
+            m, k = a.shape
+            k2, n = b.shape
+            if k2 != k or tuple(c.shape) != (m, n):
+                raise ValueError(f"problem {i}: shapes don't chain")
+            for t, nm in ((a, "a"), (b, "b"), (c, "c")):
+                if not t.is_cuda or t.dtype != torch.float32 or not t.is_contiguous():
+                    raise ValueError(f"problem {i}: {nm} must be a contiguous float32 GPU tensor")
+            arr[i] = _lib.CGemmProblem(a.data_ptr(), b.data_ptr(), c.data_ptr(), m, n, k)
+            self._keep.extend([a, b, c])
+        self._plan = ctypes.c_void_p()
+        check(lib().ef_grouped_plan_create(arr, len(problems), PRECISION[precision], int(tile), _stream(),
+                                           ctypes.byref(self._plan)))
+
+    @property
+    def tile(self) -> int:
+        return lib().ef_grouped_plan_tile(self._plan)
+
+    def run(self):
+        check(lib().ef_grouped_plan_run(self._plan, _stream()))
+
+    def close(self):
+        if self._plan:
+            lib().ef_grouped_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

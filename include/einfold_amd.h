@@ -338,6 +338,28 @@ ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const
                              const float *row_max, const float *row_sum, void *scratch,
                              void *dq, void *stream);
 
+/* Many uneven FP32 products in one persistent launch (batched dense folds
+ * with Float32 factors, dense.rs:706-770): c = a b per problem, row-major
+ * a [m][k], b [k][n], c [m][n], device pointers.  A plan prepares the
+ * problems once (work list, operand buffers, the tile shape its autotuner
+ * measured fastest for this set of shapes unless `tile` is 64 or 128) and
+ * captures prep + GEMM as a HIP graph; each run replays it on `stream`.
+ * The plan reads the problems' pointers at every run and must be rebuilt
+ * when they change.  precision: EF_PRECISION_FP32, _BF16X3 or _BF16. */
+typedef struct ef_gemm_problem {
+    const float *a;
+    const float *b;
+    float *c;
+    size_t m, n, k;
+} ef_gemm_problem;
+typedef struct ef_grouped_plan ef_grouped_plan;
+ef_status ef_grouped_plan_create(const ef_gemm_problem *problems, size_t count, uint32_t precision, int tile,
+                                 void *stream, ef_grouped_plan **plan);
+ef_status ef_grouped_plan_run(ef_grouped_plan *plan, void *stream);
+/* The plan's tile edge (64 or 128). */
+int ef_grouped_plan_tile(const ef_grouped_plan *plan);
+void ef_grouped_plan_destroy(ef_grouped_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif
