@@ -38,6 +38,7 @@ constexpr int BW = 32 * WAVES;  // rows a workgroup owns
 constexpr int PAD = 128;        // prepared operands are padded to whole 128-row blocks
 constexpr double LOG2E = 1.4426950408889634;   // cast to the precision's scalar
 constexpr double LN2 = 0.6931471805599453;
+constexpr double TAU = 8.0;     // forward weights stay below 2^TAU (log2 units)
 
 struct Span { int64_t lo, hi; };
 
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
             for (int kc = 0; kc < KC; kc++) {
                 In tmp[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) tmp[j] = x[qb] < a.tq ? Q[x[qb] * D + kc * 32 + g * 8 + j] : In(0);
+                for (int j = 0; j < 8; j++) tmp[j] = x[qb] < a.tq ? (In)(Q[x[qb] * D + kc * 32 + g * 8 + j] * scale2) : In(0);
                 make_frag<P>(qf[qb][kc], tmp);
             }
         }
@@ -317,11 +318,12 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
             else jbeg = cut < last ? cut : last;
         }
 
-        S m[2], l[2];
+        S m[2], mt[2], l[2];
         AccV o[2][DB];
 #pragma unroll
         for (int qb = 0; qb < 2; qb++) {
             m[qb] = -INFINITY;
+            mt[qb] = -INFINITY;
             l[qb] = 0;
 #pragma unroll
             for (int i = 0; i < DB; i++) o[qb][i] = AccV{0, 0, 0, 0};
@@ -398,57 +400,72 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
 #pragma unroll
                         for (int qb = 0; qb < 2; qb++) mma_part<P>(s[qb][kb], kf[kb], qf[qb][kc], sp_);
             }
-            // Online softmax per query column, the weights written over the
-            // scores.
+            // Online softmax per query column (the query fragments carry
+            // scale * log2(e), so the scores are in log2 units), the weights
+            // written over the scores.  Branch-free, both query blocks
+            // together: a column that has seen no key yet (max -inf) gets
+            // weights exp2(-inf - 0) = 0 and keeps l = 0.
+            // Masked tiles only: keys outside a query's span get -inf.
+            S mk[2][4];
+            if (full_wg) {
+#pragma unroll
+                for (int qb = 0; qb < 2; qb++)
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++)
+                        mk[qb][kb] = fmax(fmax(s[qb][kb][0], s[qb][kb][1]), fmax(s[qb][kb][2], s[qb][kb][3]));
+            } else {
+#pragma unroll
+                for (int qb = 0; qb < 2; qb++) {
+                    const int lo = (int)(sp[qb].lo - j0 < 0 ? -1 : (sp[qb].lo - j0 > BN ? BN : sp[qb].lo - j0));
+                    const int hi = (int)(sp[qb].hi - j0 < 0 ? 0 : (sp[qb].hi - j0 > BN ? BN : sp[qb].hi - j0));
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int j = kb * 16 + crow<P>(g, i);
+                            if (!(j >= lo && j < hi)) s[qb][kb][i] = S(-INFINITY);
+                        }
+                        mk[qb][kb] = fmax(fmax(s[qb][kb][0], s[qb][kb][1]), fmax(s[qb][kb][2], s[qb][kb][3]));
+                    }
+                }
+            }
+            // The weights are taken against a reference max m that moves
+            // only when a column's tile max passes it by more than TAU (all
+            // weights then stay below 2^TAU); mt keeps the true max.  When
+            // no column of the wave moves, o and l need no rescaling.
+            S tmax[2];
+            bool move = false;
 #pragma unroll
             for (int qb = 0; qb < 2; qb++) {
-                S mx = -INFINITY;
-                if (full_wg) {
+                tmax[qb] = wave_max(fmax(fmax(mk[qb][0], mk[qb][1]), fmax(mk[qb][2], mk[qb][3])));
+                mt[qb] = fmax(mt[qb], tmax[qb]);
+                move |= tmax[qb] > m[qb] + S(TAU);
+            }
+            if (__any(move)) {
 #pragma unroll
-                    for (int kb = 0; kb < 4; kb++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const S t = s[qb][kb][i] * scale2;
-                            s[qb][kb][i] = t;
-                            mx = fmax(mx, t);
-                        }
-                } else {
-                    const int64_t lo = sp[qb].lo, hi = sp[qb].hi;
-#pragma unroll
-                    for (int kb = 0; kb < 4; kb++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const int64_t j = j0 + kb * 16 + crow<P>(g, i);
-                            const S t = j >= lo && j < hi ? s[qb][kb][i] * scale2 : S(-INFINITY);
-                            s[qb][kb][i] = t;
-                            mx = fmax(mx, t);
-                        }
-                }
-                mx = wave_max(mx);
-                const S mnew = fmax(mx, m[qb]);
-                S rs = 0;
-                if (mnew == -INFINITY) {
-#pragma unroll
-                    for (int kb = 0; kb < 4; kb++) s[qb][kb] = AccV{0, 0, 0, 0};
-                } else {
-#pragma unroll
-                    for (int kb = 0; kb < 4; kb++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            s[qb][kb][i] = fast_exp2(s[qb][kb][i] - mnew);
-                            rs += s[qb][kb][i];
-                        }
-                }
-                rs = group_sum(rs);
-                if (mnew != -INFINITY) {
-                    const S alpha = fast_exp2(m[qb] - mnew);   // 0 when m was -inf
-                    l[qb] = l[qb] * alpha + rs;
+                for (int qb = 0; qb < 2; qb++) {
+                    const S mnew = tmax[qb] > m[qb] + S(TAU) ? tmax[qb] : m[qb];
+                    // 1 when unmoved, 0 when m was -inf (also when it stays so)
+                    const S alpha = fast_exp2(m[qb] - (mnew == -INFINITY ? S(0) : mnew));
+                    m[qb] = mnew;
+                    l[qb] *= alpha;
 #pragma unroll
                     for (int db = 0; db < DB; db++)
 #pragma unroll
                         for (int i = 0; i < 4; i++) o[qb][db][i] *= alpha;
-                    m[qb] = mnew;
                 }
+            }
+#pragma unroll
+            for (int qb = 0; qb < 2; qb++) {
+                const S msub = m[qb] == -INFINITY ? S(0) : m[qb];   // no key seen yet: weights 0
+                S rk[4];
+#pragma unroll
+                for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) s[qb][kb][i] = fast_exp2(s[qb][kb][i] - msub);
+                    rk[kb] = (s[qb][kb][0] + s[qb][kb][1]) + (s[qb][kb][2] + s[qb][kb][3]);
+                }
+                l[qb] += group_sum((rk[0] + rk[1]) + (rk[2] + rk[3]));
             }
             // O^T += V^T P^T, P^T's 32-key chunks from two stacked S blocks.
 #pragma unroll
@@ -481,6 +498,17 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
                 if (lane == 0) flags[0][w] = fn;
                 __syncthreads();
             }
+        }
+        // The sums rescaled from the reference max to the true max.
+#pragma unroll
+        for (int qb = 0; qb < 2; qb++) {
+            const S beta = m[qb] == -INFINITY ? S(1) : fast_exp2(m[qb] - mt[qb]);
+            m[qb] = mt[qb];
+            l[qb] *= beta;
+#pragma unroll
+            for (int db = 0; db < DB; db++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) o[qb][db][i] *= beta;
         }
         // out[x][d] for d = db*16 + crow(g, i): the weighted sums over the
         // normalizer (a query with no keys has l = 0 and gets 0/0 = NaN).

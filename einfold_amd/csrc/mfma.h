@@ -30,6 +30,8 @@ enum : int { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2, PREC_BF16 = 3 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -120,12 +122,18 @@ template <int P, class T> __device__ __forceinline__ void make_frag(Frag<P> &f, 
 #pragma unroll
         for (int j = 0; j < 8; j++) f.hi[j] = (__bf16)(float)x[j];
     } else {
+        // Two values per conversion; hi read back from the packed bits.
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            __bf16 h, l;
-            split_bf16((float)x[j], h, l);
-            f.hi[j] = h;
-            f.lo[j] = l;
+        for (int j = 0; j < 8; j += 2) {
+            const float x0 = (float)x[j], x1 = (float)x[j + 1];
+            const bf16x2 h = __builtin_convertvector((f32x2){x0, x1}, bf16x2);
+            const unsigned u = __builtin_bit_cast(unsigned, h);
+            const f32x2 r = {x0 - __uint_as_float(u << 16), x1 - __uint_as_float(u & 0xffff0000u)};
+            const bf16x2 l = __builtin_convertvector(r, bf16x2);
+            f.hi[j] = h[0];
+            f.hi[j + 1] = h[1];
+            f.lo[j] = l[0];
+            f.lo[j + 1] = l[1];
         }
     }
 }
@@ -192,20 +200,53 @@ __device__ __forceinline__ void tile_load_cperm(Frag<P> &f, const typename Elem<
     }
 }
 
-template <class T> __device__ __forceinline__ T wave_max(T x) {
-    T y = __shfl_xor(x, 16);
-    x = x > y ? x : y;
-    y = __shfl_xor(x, 32);
-    x = x > y ? x : y;
-    return x;
+// The values of lanes i % 32 and 32 + i % 32 (a, b), and of lanes
+// 32 * (i / 32) + i % 16 and that + 16 (rows 0/1 or 2/3 of the wave), by
+// gfx950's lane-swap instructions: no LDS round trip.
+__device__ __forceinline__ void lanes32(float x, float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void lanes16(float x, float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void lanes32(double x, double &a, double &b) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    a = __hiloint2double((int)rh[0], (int)rl[0]);
+    b = __hiloint2double((int)rh[1], (int)rl[1]);
+}
+__device__ __forceinline__ void lanes16(double x, double &a, double &b) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    a = __hiloint2double((int)rh[0], (int)rl[0]);
+    b = __hiloint2double((int)rh[1], (int)rl[1]);
 }
 
-// Sum over the four lane groups that share a column (l % 16), in a fixed
-// order: (g0 + g1) + (g2 + g3) on every lane.
+// Max over the four lane groups that share a column (l % 16), the same on
+// every lane of the column (NaN is ignored unless all four are NaN).
+template <class T> __device__ __forceinline__ T wave_max(T x) {
+    T a, b;
+    lanes16(x, a, b);
+    x = fmax(a, b);
+    lanes32(x, a, b);
+    return fmax(a, b);
+}
+
+// Sum over the four lane groups that share a column, in a fixed order:
+// (g0 + g1) + (g2 + g3) on every lane (addition commutes, so each lane's
+// pairs give the same bits).
 template <class T> __device__ __forceinline__ T group_sum(T x) {
-    x = x + __shfl_xor(x, 16);
-    x = x + __shfl_xor(x, 32);
-    return x;
+    T a, b;
+    lanes16(x, a, b);
+    x = a + b;
+    lanes32(x, a, b);
+    return a + b;
 }
 
 }  // namespace ef
