@@ -36,11 +36,16 @@ constexpr int WAVES = 4;        // waves per workgroup, 32 owned rows each
 constexpr int THREADS = 64 * WAVES;
 constexpr int BW = 32 * WAVES;  // rows a workgroup owns
 constexpr int PAD = 128;        // prepared operands are padded to whole 128-row blocks
+// Row pitch padding (elements) of LDS tiles read by 16-byte fragment
+// loads, one row per lane: 32 bytes for BF16 rows (a 16-byte pad leaves
+// those loads two-way bank-conflicted on gfx950), 8 elements otherwise.
+template <class E> constexpr int RPAD = sizeof(E) == 2 ? 16 : 8;
 constexpr double LOG2E = 1.4426950408889634;   // cast to the precision's scalar
 constexpr double LN2 = 0.6931471805599453;
 constexpr double TAU = 8.0;     // forward weights stay below 2^TAU (log2 units)
 
 struct Span { int64_t lo, hi; };
+struct Span32 { int lo, hi; };
 
 struct Mask {
     const int64_t *allowed;
@@ -185,7 +190,8 @@ template <class E, int PLN, int R, int C> struct TileIO {
 };
 
 // The union of the spans of a workgroup's queries (two per lane).
-__device__ __forceinline__ void group_range(const Span (&sp)[2], int64_t *red_lo, int64_t *red_hi, int64_t &first,
+template <class SP>
+__device__ __forceinline__ void group_range(const SP (&sp)[2], int64_t *red_lo, int64_t *red_hi, int64_t &first,
                                             int64_t &last) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int64_t lo = INT64_MAX, hi = 0;
@@ -220,39 +226,46 @@ struct FwdArgs {
     void *out;
     float *row_max, *row_sum;
     unsigned long long *visited;
-    Counter *ctr;
     Mask mask;
     int64_t tq, tk, tkp, batches;
     double scale;
+    Counter *ctr;
     int items, reverse, parts, pad;
     void *po, *pm, *pl;           // parts == 2: each part's unnormalized sums, max and sum
 };
 
-// The K and V^T tiles of one 64-key step, as 16-byte chunks: thread t
-// moves chunks t, t + THREADS, ...; the first KN of each thread's are K's.
+// The K and V^T tiles of one 64-key step in LDS: K rows (KP elements,
+// data then padding) then V^T rows (VP), plane after plane, filled by 1 KiB
+// global_load_lds_dwordx4 instructions (lane l writes the 16 bytes at
+// 16 l; a padding slot gets a copy of its row's first chunk, slots past
+// the tile a copy of the tile's first chunk).  Instruction n is issued by
+// wave n % WAVES.
 template <int P, int D> struct FwdTiles {
     typedef typename Elem<P>::T E;
     static constexpr int PL = Elem<P>::planes;
     static constexpr int CE = 16 / sizeof(E);
-    static constexpr int KP = D + 8, VP = BN + 8;
+    static constexpr int KP = D + RPAD<E>, VP = BN + 8;
     static constexpr int KPLANE = BN * KP, VPLANE = D * VP;
-    static constexpr int KCH = PL * BN * (D / CE);     // K chunks
-    static constexpr int VCH = PL * D * (BN / CE);     // V^T chunks
-    static constexpr int KN = KCH / THREADS, VN = VCH / THREADS, N = KN + VN;
-    static constexpr bool prefetch = N <= 16;          // hold the next tile in registers
-    static constexpr int BUF = PL * (KPLANE + VPLANE); // elements per LDS buffer
+    static constexpr int KS = KP / CE, VS = VP / CE;                 // 16-byte slots per LDS row
+    static constexpr int KSLOTS = PL * BN * KS, VSLOTS = PL * D * VS;
+    static constexpr int NK = KSLOTS / 64, NV = (VSLOTS + 63) / 64, N = NK + NV;
+    static constexpr int PER = (N + WAVES - 1) / WAVES;              // instructions per wave
+    static constexpr int BUF = N * 1024 / (int)sizeof(E);            // elements per LDS buffer
+    static constexpr int OCC = (D <= 64 && (P == PREC_BF16X3 || P == PREC_BF16)) ? 2 : 1;   // workgroups per CU
+    static_assert(KSLOTS % 64 == 0 && KP % CE == 0 && VP % CE == 0, "tile rows fill whole slots");
+    static_assert(OCC == 1 || 2 * BUF * (int)sizeof(E) <= 80 * 1024, "two workgroups' buffers fit in LDS");
 };
 
 template <int P, int D>
-__global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(FwdTiles<P, D>::OCC)))
+void attn_fwd_kernel(FwdArgs a) {
     typedef typename Input<P>::T In;
     typedef typename Acc<P>::type AccV;
     typedef typename Acc<P>::scalar S;
     typedef typename Elem<P>::T E;
     typedef FwdTiles<P, D> F;
     constexpr int KC = D / 32, DB = D / 16, NP = Parts<P>::n;
-    constexpr int NBUF = F::prefetch ? 2 : 1;
-    __shared__ __attribute__((aligned(16))) E lds[NBUF * F::BUF];
+    __shared__ __attribute__((aligned(1024))) E lds[2 * F::BUF];
     __shared__ int64_t red_lo[WAVES], red_hi[WAVES];
     __shared__ int flags[2][WAVES];   // per buffer, per wave: bit 0 any, bit 1 full
     __shared__ int slot;
@@ -263,22 +276,24 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
     const S scale2 = (S)a.scale * (S)LOG2E;   // scores in log2 units
     unsigned long long visited = 0;
 
-    // This thread's chunks: global offsets from the tile's base, LDS offsets
-    // from the buffer's (loop-invariant).
-    int goff[F::N];   // fits: a plane of prepared operands stays below 2^31 elements (checked on the host)
-    int loff[F::N];
+    // This lane's source for each of its wave's load instructions, in
+    // elements from the K tile's base (n < NK) or the V^T tile's.
+    int goff[F::PER];   // fits: a plane of prepared operands stays below 2^31 elements (checked on the host)
 #pragma unroll
-    for (int i = 0; i < F::N; i++) {
-        if (i < F::KN) {
-            const int k = tid + THREADS * i, per_row = D / F::CE, per_plane = BN * per_row;
-            const int pl = k / per_plane, r = (k % per_plane) / per_row, col = (k % per_row) * F::CE;
-            goff[i] = (int)(pl * kplane + (int64_t)r * D + col);
-            loff[i] = pl * F::KPLANE + r * F::KP + col;
-        } else {
-            const int k = tid + THREADS * (i - F::KN), per_row = BN / F::CE, per_plane = D * per_row;
-            const int pl = k / per_plane, r = (k % per_plane) / per_row, col = (k % per_row) * F::CE;
-            goff[i] = (int)(pl * vplane + (int64_t)r * a.tkp + col);
-            loff[i] = F::PL * F::KPLANE + pl * F::VPLANE + r * F::VP + col;
+    for (int i = 0; i < F::PER; i++) {
+        const int n = w + WAVES * i;
+        goff[i] = 0;
+        if (n < F::NK) {
+            const int sl = n * 64 + lane, pl = sl / (BN * F::KS), rem = sl % (BN * F::KS);
+            const int r = rem / F::KS, col = rem % F::KS < D / F::CE ? rem % F::KS : 0;
+            goff[i] = (int)(pl * kplane + (int64_t)r * D + col * F::CE);
+        } else if (n < F::N) {
+            const int sl = (n - F::NK) * 64 + lane;
+            if (sl < F::VSLOTS) {
+                const int pl = sl / (D * F::VS), rem = sl % (D * F::VS);
+                const int r = rem / F::VS, col = rem % F::VS < BN / F::CE ? rem % F::VS : 0;
+                goff[i] = (int)(pl * vplane + (int64_t)r * a.tkp + col * F::CE);
+            }
         }
     }
     for (;;) {
@@ -293,18 +308,20 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
         const E *K = static_cast<const E *>(a.kr) + b * a.tkp * D;
         const E *VT = static_cast<const E *>(a.vt) + b * D * a.tkp;
 
-        int64_t x[2];
-        Span sp[2];
+        // Query rows and spans in 32 bits (the host keeps tq, tk < 2^31).
+        int x[2];
+        Span32 sp[2];
         Frag<P> qf[2][KC];
 #pragma unroll
         for (int qb = 0; qb < 2; qb++) {
-            x[qb] = q0 + w * 32 + qb * 16 + c;
-            sp[qb] = query_span(a.mask, x[qb]);
+            x[qb] = (int)q0 + w * 32 + qb * 16 + c;
+            const Span s64 = query_span(a.mask, x[qb]);
+            sp[qb] = {(int)s64.lo, (int)s64.hi};
 #pragma unroll
             for (int kc = 0; kc < KC; kc++) {
                 In tmp[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) tmp[j] = x[qb] < a.tq ? (In)(Q[x[qb] * D + kc * 32 + g * 8 + j] * scale2) : In(0);
+                for (int j = 0; j < 8; j++) tmp[j] = x[qb] < a.tq ? (In)(Q[(int64_t)x[qb] * D + kc * 32 + g * 8 + j] * scale2) : In(0);
                 make_frag<P>(qf[qb][kc], tmp);
             }
         }
@@ -339,37 +356,30 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
         }                                                                                        \
         return (__any(any_) ? 1 : 0) | (__all(full_) ? 2 : 0);                                   \
     }())
-#define EF_LOAD_TILE(J0)                                                                         \
-    _Pragma("unroll") for (int i_ = 0; i_ < F::N; i_++)                                          \
-        pre[i_] = *reinterpret_cast<const u32x4 *>((i_ < F::KN ? K + (J0) * D : VT + (J0)) + goff[i_])
-#define EF_STORE_TILE(BUF)                                                                       \
-    _Pragma("unroll") for (int i_ = 0; i_ < F::N; i_++)                                          \
-        *reinterpret_cast<u32x4 *>((BUF) + loff[i_]) = pre[i_]
+// Issues this wave's loads of the tile at J0 into LDS buffer BUF.
+#define EF_LOAD_TILE(J0, BUF)                                                                    \
+    _Pragma("unroll") for (int i_ = 0; i_ < F::PER; i_++) {                                      \
+        const int n_ = w + WAVES * i_;                                                           \
+        if (n_ < F::N)                                                                           \
+            __builtin_amdgcn_global_load_lds(                                                    \
+                (const void *)((n_ < F::NK ? K + (J0) * D : VT + (J0)) + goff[i_]),              \
+                (__attribute__((address_space(3))) void *)((char *)(BUF) + n_ * 1024), 16, 0, 0);     \
+    }
         if (jbeg < last) {
-            if (F::prefetch) {
-                u32x4 pre[F::N];
-                EF_LOAD_TILE(jbeg);
-                EF_STORE_TILE(lds);
-            }
+            EF_LOAD_TILE(jbeg, lds);
             const int f0 = EF_WAVE_FLAGS(jbeg);
             if (lane == 0) flags[0][w] = f0;
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
         int cur = 0;
         for (int64_t j0 = jbeg; j0 < last; j0 += BN) {
             const bool more = j0 + BN < last;
-            u32x4 pre[F::N];
-            if (F::prefetch) {
-                if (more) {
-                    EF_LOAD_TILE(j0 + BN);
-                    const int fn = EF_WAVE_FLAGS(j0 + BN);
-                    if (lane == 0) flags[cur ^ 1][w] = fn;
-                }
-            } else {
-                // One buffer: stage this tile now.
-                stage<E>(lds, F::KP, F::KPLANE, K + j0 * D, D, kplane, BN, D, F::PL);
-                stage<E>(lds + F::PL * F::KPLANE, F::VP, F::VPLANE, VT + j0, a.tkp, vplane, D, BN, F::PL);
-                __syncthreads();
+            if (more) {
+                // The next tile streams into the other buffer meanwhile.
+                EF_LOAD_TILE(j0 + BN, lds + (cur ^ 1) * F::BUF);
+                const int fn = EF_WAVE_FLAGS(j0 + BN);
+                if (lane == 0) flags[cur ^ 1][w] = fn;
             }
             int f_and = 3;
 #pragma unroll
@@ -380,7 +390,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
             if (tid == 0)
 #pragma unroll
                 for (int h = 0; h < WAVES / 2; h++) visited += (unsigned)(((flags[cur][2 * h] | flags[cur][2 * h + 1]) & 1) != 0);
-            const E *ks = lds + (F::prefetch ? cur * F::BUF : 0);
+            const E *ks = lds + cur * F::BUF;
             const E *vt = ks + F::PL * F::KPLANE;
 
             AccV s[2][4];
@@ -390,15 +400,19 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
                 for (int kb = 0; kb < 4; kb++) s[qb][kb] = AccV{0, 0, 0, 0};
 #pragma unroll
             for (int kc = 0; kc < KC; kc++) {
-                Frag<P> kf[4];
 #pragma unroll
-                for (int kb = 0; kb < 4; kb++) tile_load8<P>(kf[kb], ks, F::KPLANE, (kb * 16 + c) * F::KP + kc * 32 + g * 8);
+                for (int kh = 0; kh < 4; kh += 2) {   // two key blocks' fragments live at a time
+                    Frag<P> kf[2];
 #pragma unroll
-                for (int sp_ = 0; sp_ < NP; sp_++)
+                    for (int kb = 0; kb < 2; kb++)
+                        tile_load8<P>(kf[kb], ks, F::KPLANE, ((kh + kb) * 16 + c) * F::KP + kc * 32 + g * 8);
 #pragma unroll
-                    for (int kb = 0; kb < 4; kb++)
+                    for (int sp_ = 0; sp_ < NP; sp_++)
 #pragma unroll
-                        for (int qb = 0; qb < 2; qb++) mma_part<P>(s[qb][kb], kf[kb], qf[qb][kc], sp_);
+                        for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                            for (int qb = 0; qb < 2; qb++) mma_part<P>(s[qb][kh + kb], kf[kb], qf[qb][kc], sp_);
+                }
             }
             // Online softmax per query column (the query fragments carry
             // scale * log2(e), so the scores are in log2 units), the weights
@@ -478,26 +492,25 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
                     for (int i = 0; i < 4; i++) { tmp[i] = s[qb][2 * kc][i]; tmp[i + 4] = s[qb][2 * kc + 1][i]; }
                     make_frag<P>(pf[qb], tmp);
                 }
-                Frag<P> vf[DB];
 #pragma unroll
-                for (int db = 0; db < DB; db++) tile_load_cperm<P>(vf[db], vt, F::VPLANE, (db * 16 + c) * F::VP + kc * 32, g);
+                for (int dh = 0; dh < DB; dh += 2) {
+                    Frag<P> vf[2];
 #pragma unroll
-                for (int sp_ = 0; sp_ < NP; sp_++)
+                    for (int db = 0; db < 2; db++)
+                        tile_load_cperm<P>(vf[db], vt, F::VPLANE, ((dh + db) * 16 + c) * F::VP + kc * 32, g);
 #pragma unroll
-                    for (int db = 0; db < DB; db++)
+                    for (int sp_ = 0; sp_ < NP; sp_++)
 #pragma unroll
-                        for (int qb = 0; qb < 2; qb++) mma_part<P>(o[qb][db], vf[db], pf[qb], sp_);
+                        for (int db = 0; db < 2; db++)
+#pragma unroll
+                            for (int qb = 0; qb < 2; qb++) mma_part<P>(o[qb][dh + db], vf[db], pf[qb], sp_);
+                }
             }
-            if (F::prefetch && more) {
-                EF_STORE_TILE(lds + (cur ^ 1) * F::BUF);
-            }
+            // This wave's loads have landed; after the barrier, everyone's
+            // have, and no wave still reads this buffer.
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            if (F::prefetch) cur ^= 1;
-            else if (more) {
-                const int fn = EF_WAVE_FLAGS(j0 + BN);
-                if (lane == 0) flags[0][w] = fn;
-                __syncthreads();
-            }
+            cur ^= 1;
         }
         // The sums rescaled from the reference max to the true max.
 #pragma unroll
@@ -547,7 +560,6 @@ __global__ __launch_bounds__(THREADS, 1) void attn_fwd_kernel(FwdArgs a) {
 }
 #undef EF_WAVE_FLAGS
 #undef EF_LOAD_TILE
-#undef EF_STORE_TILE
 
 // The two parts of each query's keys combined, part 0 then part 1: the
 // sums rescaled to the larger max (log2 units), then normalized.
@@ -663,7 +675,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     typedef typename Elem<P>::T E;
     constexpr int PL = Elem<P>::planes;
     constexpr int KC = D / 32, DB = D / 16;
-    constexpr int RP = D + 8, TP = BN + 8;
+    constexpr int RP = D + RPAD<E>, TP = BN + 8;
     constexpr int RPLANE = BN * RP, TPLANE = D * TP;
     __shared__ __attribute__((aligned(16))) E qs[PL * RPLANE];
     __shared__ __attribute__((aligned(16))) E qtr[PL * TPLANE];
@@ -914,7 +926,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     typedef typename Elem<P>::T E;
     constexpr int PL = Elem<P>::planes;
     constexpr int KC = D / 32, DB = D / 16;
-    constexpr int RP = D + 8, TP = BN + 8;
+    constexpr int RP = D + RPAD<E>, TP = BN + 8;
     constexpr int RPLANE = BN * RP, TPLANE = D * TP;
     __shared__ __attribute__((aligned(16))) E ks[PL * RPLANE];
     __shared__ __attribute__((aligned(16))) E ktr[PL * TPLANE];
@@ -1125,6 +1137,7 @@ bool fast_fwd_shape(const ef_attention_desc &d) {
     if (d.features != d.values) return false;
     // The kernel's 32-bit offsets into a plane of prepared operands.
     if (2 * d.batches * (size_t)round_up((int64_t)d.keys, PAD) * d.features >= (size_t(1) << 31)) return false;
+    if (d.queries >= (size_t(1) << 31) - BW) return false;   // 32-bit query rows
     if (d.precision == EF_PRECISION_FP64) return d.features == 32 || d.features == 64;
     return d.features == 32 || d.features == 64 || d.features == 128;
 }
@@ -1135,6 +1148,7 @@ int parts_for(const ef_attention_desc &d, int64_t blocks) {
     const bool varying = d.allowed || d.causal;
     return varying && blocks < 4 * (int64_t)compute_units() ? 2 : 1;
 }
+
 
 bool fast_bwd_shape(const ef_attention_desc &d) {
     return d.features == d.values && (d.features == 32 || d.features == 64);
@@ -1209,10 +1223,10 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
     a.q = q; a.kr = kr; a.vt = vt; a.out = out;
     a.row_max = row_max; a.row_sum = row_sum;
     a.visited = reinterpret_cast<unsigned long long *>(visited);
-    a.ctr = reinterpret_cast<Counter *>(scratch + L.counters);
     a.mask = make_mask(d);
     a.tq = tq; a.tk = tk; a.tkp = tkp; a.batches = B;
     a.scale = d.scale;
+    a.ctr = reinterpret_cast<Counter *>(scratch + L.counters);
     const int64_t blocks = B * ((tq + BW - 1) / BW);
     a.parts = parts_for(d, blocks);
     a.pad = 0;
