@@ -97,14 +97,60 @@ __device__ __forceinline__ void put_split(typename Elem<P>::T *base, int64_t pla
 // transposed planes [planes][B][D][Tp], zero past T.  One block per 64
 // rows of one batch.
 // ---------------------------------------------------------------------
+struct Counter { unsigned next, done; };
+
+// One prep launch: blockIdx.z picks a job.  Job z < njobs prepares one
+// operand; job z == njobs (when out is set) computes delta[b][x] = sum_d
+// dout * out for 64 rows per block, a fixed-order tree over 16 lanes per
+// row.  Block (0, 0, 0) also zeroes the work counters the launch's
+// kernels will take items from.
+struct PrepJob {
+    const void *x;
+    int64_t T, Tp;
+    void *rows, *trans;
+};
+struct PrepArgs {
+    PrepJob job[4];
+    int njobs, nzero;
+    int64_t B;
+    Counter *zero;
+    const void *out, *dout;
+    float *delta;
+    int64_t tq;
+};
+
 template <int P, int D>
-__global__ __launch_bounds__(256) void prep_kernel(const typename Input<P>::T *__restrict__ x, int64_t T, int64_t Tp,
-                                                   int64_t B, typename Elem<P>::T *rows, typename Elem<P>::T *trans) {
+__global__ __launch_bounds__(256) void prep_kernel(PrepArgs a) {
     typedef typename Input<P>::T In;
+    typedef typename Elem<P>::T E;
     __shared__ In t[64][D + 1];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && (int)threadIdx.x < a.nzero) {
+        a.zero[threadIdx.x].next = 0;
+        a.zero[threadIdx.x].done = 0;
+    }
     const int64_t b = blockIdx.y;
     const int64_t t0 = (int64_t)blockIdx.x * 64;
-    const int64_t plane = B * Tp * D;
+    if ((int)blockIdx.z == a.njobs) {
+        // delta rows t0 .. t0 + 63 of batch b
+        const In *out = static_cast<const In *>(a.out), *dout = static_cast<const In *>(a.dout);
+        const int c = threadIdx.x & 15;
+        for (int it = 0; it < 4; it++) {
+            const int64_t x = t0 + it * 16 + (threadIdx.x >> 4);
+            const int64_t r = b * a.tq + x;
+            float sm = 0;
+            if (x < a.tq)
+                for (int e = c; e < D; e += 16) sm += (float)out[r * D + e] * (float)dout[r * D + e];
+            for (int off = 8; off >= 1; off >>= 1) sm += __shfl_xor(sm, off, 16);
+            if (x < a.tq && c == 0) a.delta[r] = sm;
+        }
+        return;
+    }
+    const PrepJob &j = a.job[blockIdx.z];
+    if (t0 >= j.Tp) return;
+    const In *x = static_cast<const In *>(j.x);
+    E *rows = static_cast<E *>(j.rows), *trans = static_cast<E *>(j.trans);
+    const int64_t T = j.T, Tp = j.Tp;
+    const int64_t plane = a.B * Tp * D;
     for (int i = threadIdx.x; i < 64 * D; i += 256) {
         const int r = i / D, c = i % D;
         const int64_t tt = t0 + r;
@@ -124,8 +170,6 @@ __global__ __launch_bounds__(256) void prep_kernel(const typename Input<P>::T *_
 // Work counters: one per kernel, zeroed by a small kernel before each
 // launch (plain stores, so the pair may be captured in a graph).
 // ---------------------------------------------------------------------
-struct Counter { unsigned next, done; };
-
 __global__ void zero_counters_kernel(Counter *c) {
     if (threadIdx.x < 4) { c[threadIdx.x].next = 0; c[threadIdx.x].done = 0; }
 }
@@ -581,11 +625,19 @@ __global__ __launch_bounds__(256) void attn_merge_kernel(const S *po, const S *p
     }
 }
 
-// out = (p0 + p1) * scale, part 0 first.
+// out = (p0 + p1) * scale, part 0 first; a second array (p2, scale2, out2)
+// of the same size in the same launch.
 template <class S, class In>
-__global__ __launch_bounds__(256) void attn_sum_kernel(const S *p, int64_t n, double scale, In *out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (In)((p[i] + p[n + i]) * (S)scale);
+__global__ __launch_bounds__(256) void attn_sum_kernel(const S *p, int64_t n, double scale, In *out, const S *p2,
+                                                       double scale2, In *out2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {   // the second array, if any, in the grid's upper half
+        i -= n;
+        p = p2;
+        scale = scale2;
+        out = out2;
+    }
+    if (p && i < n) out[i] = (In)((p[i] + p[n + i]) * (S)scale);
 }
 
 // Any shape, one thread per (query, block of 8 value columns), keys in
@@ -653,17 +705,6 @@ struct BwdArgs {
     void *pdk, *pdv, *pdq;             // parts == 2: each part's unscaled sums
 };
 
-// delta[b][x] = sum_d dout[b][x][d] * out[b][x][d], a fixed-order tree.
-template <class In, int D>
-__global__ __launch_bounds__(256) void attn_delta_kernel(const In *out, const In *dout, float *delta, int64_t rows) {
-    const int64_t r = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    const int c = threadIdx.x & 15;
-    float s = 0;
-    if (r < rows)
-        for (int e = c; e < D; e += 16) s += (float)out[r * D + e] * (float)dout[r * D + e];
-    for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 16);
-    if (r < rows && c == 0) delta[r] = s;
-}
 
 // dK and dV for 128 keys per workgroup (32 per wave), visiting the query
 // tiles that may see them.
@@ -1216,9 +1257,14 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
     const int64_t tkp = round_up(tk, PAD);
     E *kr = reinterpret_cast<E *>(scratch + L.kr);
     E *vt = reinterpret_cast<E *>(scratch + L.vt);
-    dim3 pg((unsigned)(tkp / 64), (unsigned)B);
-    hipLaunchKernelGGL((prep_kernel<P, D>), pg, dim3(256), 0, s, (const In *)k, tk, tkp, B, kr, (E *)nullptr);
-    hipLaunchKernelGGL((prep_kernel<P, D>), pg, dim3(256), 0, s, (const In *)v, tk, tkp, B, (E *)nullptr, vt);
+    PrepArgs pa = {};
+    pa.job[0] = {k, tk, tkp, kr, nullptr};
+    pa.job[1] = {v, tk, tkp, nullptr, vt};
+    pa.njobs = 2;
+    pa.B = B;
+    pa.zero = reinterpret_cast<Counter *>(scratch + L.counters);
+    pa.nzero = 1;
+    hipLaunchKernelGGL((prep_kernel<P, D>), dim3((unsigned)(tkp / 64), (unsigned)B, 2), dim3(256), 0, s, pa);
     FwdArgs a;
     a.q = q; a.kr = kr; a.vt = vt; a.out = out;
     a.row_max = row_max; a.row_sum = row_sum;
@@ -1235,7 +1281,6 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
     a.po = scratch + L.po; a.pm = scratch + L.pm; a.pl = scratch + L.pl;
     const int cap = 2 * compute_units();
     const int grid = a.items < cap ? a.items : cap;
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
     hipLaunchKernelGGL((attn_fwd_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
     if (a.parts == 2) {
         typedef typename Acc<P>::scalar S;
@@ -1285,16 +1330,23 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
     a.pdk = scratch + L.pdk; a.pdv = scratch + L.pdv; a.pdq = scratch + L.pdq;
     typedef typename Acc<P>::scalar S;
     if (stages & 1) {
-        dim3 gq((unsigned)(tqp / 64), (unsigned)B), gk((unsigned)(tkp / 64), (unsigned)B);
-        hipLaunchKernelGGL((prep_kernel<P, D>), gq, dim3(256), 0, s, (const In *)q, tq, tqp, B, (E *)a.qr, (E *)a.qt);
-        hipLaunchKernelGGL((prep_kernel<P, D>), gq, dim3(256), 0, s, (const In *)dout, tq, tqp, B, (E *)a.dor,
-                           (E *)a.dot);
-        hipLaunchKernelGGL((prep_kernel<P, D>), gk, dim3(256), 0, s, (const In *)k, tk, tkp, B, (E *)a.kr, (E *)a.kt);
-        hipLaunchKernelGGL((prep_kernel<P, D>), gk, dim3(256), 0, s, (const In *)v, tk, tkp, B, (E *)a.vr,
-                           (E *)nullptr);
-        const int64_t rows = B * tq;
-        hipLaunchKernelGGL((attn_delta_kernel<In, D>), dim3((unsigned)((rows + 15) / 16)), dim3(256), 0, s,
-                           (const In *)out, (const In *)dout, (float *)a.delta, rows);
+        // Q, dO, K and V prepared, delta computed and both work counters
+        // zeroed, in one launch.
+        PrepArgs pa = {};
+        pa.job[0] = {q, tq, tqp, scratch + L.qr, scratch + L.qt};
+        pa.job[1] = {dout, tq, tqp, scratch + L.dor, scratch + L.dot};
+        pa.job[2] = {k, tk, tkp, scratch + L.bkr, scratch + L.bkt};
+        pa.job[3] = {v, tk, tkp, scratch + L.bvr, nullptr};
+        pa.njobs = 4;
+        pa.B = B;
+        pa.zero = reinterpret_cast<Counter *>(scratch + L.counters);
+        pa.nzero = 2;
+        pa.out = out;
+        pa.dout = dout;
+        pa.delta = (float *)a.delta;
+        pa.tq = tq;
+        const int64_t xb = (tqp > tkp ? tqp : tkp) / 64;
+        hipLaunchKernelGGL((prep_kernel<P, D>), dim3((unsigned)xb, (unsigned)B, 5), dim3(256), 0, s, pa);
     }
     const int cap = 2 * compute_units();
     if (stages & 2) {
@@ -1304,14 +1356,12 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         a.items = (int)(blocks * a.parts);
         a.reverse = 0;   // key tiles in order: under a causal mask the first are the heaviest
         const int grid = a.items < cap ? a.items : cap;
-        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
+        if (!(stages & 1)) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
         hipLaunchKernelGGL((attn_bwd_dkdv_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
         if (a.parts == 2) {
             const int64_t n = B * tk * D;
-            hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const S *)a.pdk, n,
-                               d.scale, (In *)dk);
-            hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const S *)a.pdv, n,
-                               1.0, (In *)dv);
+            hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(2 * n, 256)), dim3(256), 0, s,
+                               (const S *)a.pdk, n, d.scale, (In *)dk, (const S *)a.pdv, 1.0, (In *)dv);
         }
     }
     if (stages & 4) {
@@ -1321,12 +1371,12 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         a.items = (int)(blocks * a.parts);
         a.reverse = d.causal ? 1 : 0;   // under a causal mask the last query tiles are the heaviest
         const int grid = a.items < cap ? a.items : cap;
-        hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
+        if (!(stages & 1)) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
         hipLaunchKernelGGL((attn_bwd_dq_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
         if (a.parts == 2) {
             const int64_t n = B * tq * D;
             hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const S *)a.pdq, n,
-                               d.scale, (In *)dq);
+                               d.scale, (In *)dq, (const S *)nullptr, 0.0, (In *)nullptr);
         }
     }
     EF_HIP(hipGetLastError());
