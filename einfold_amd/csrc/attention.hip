@@ -354,18 +354,24 @@ void attn_fwd_kernel(FwdArgs a) {
 
         // Query rows and spans in 32 bits (the host keeps tq, tk < 2^31).
         int x[2];
+        int64_t xr[2];
         Span32 sp[2];
         Frag<P> qf[2][KC];
 #pragma unroll
         for (int qb = 0; qb < 2; qb++) {
             x[qb] = (int)q0 + w * 32 + qb * 16 + c;
+            // Rows past tq load row tq - 1 (and use 0): unconditional loads
+            // issue together instead of one branch and wait each.
+            xr[qb] = x[qb] < a.tq ? x[qb] : a.tq - 1;
             const Span s64 = query_span(a.mask, x[qb]);
             sp[qb] = {(int)s64.lo, (int)s64.hi};
 #pragma unroll
             for (int kc = 0; kc < KC; kc++) {
                 In tmp[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) tmp[j] = x[qb] < a.tq ? (In)(Q[(int64_t)x[qb] * D + kc * 32 + g * 8 + j] * scale2) : In(0);
+                for (int j = 0; j < 8; j++) tmp[j] = Q[xr[qb] * D + kc * 32 + g * 8 + j];
+#pragma unroll
+                for (int j = 0; j < 8; j++) tmp[j] = x[qb] < a.tq ? (In)(tmp[j] * scale2) : In(0);
                 make_frag<P>(qf[qb][kc], tmp);
             }
         }
@@ -759,9 +765,15 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
             for (int kc = 0; kc < KC; kc++) {
                 In t1[8], t2[8];
 #pragma unroll
+                for (int j = 0; j < 8; j++) {   // unconditional loads (row clamped), then 0 past tk
+                    const int64_t kr = kj[kb] < a.tk ? kj[kb] : a.tk - 1;
+                    t1[j] = K[kr * D + kc * 32 + g * 8 + j];
+                    t2[j] = V[kr * D + kc * 32 + g * 8 + j];
+                }
+#pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    t1[j] = kj[kb] < a.tk ? K[kj[kb] * D + kc * 32 + g * 8 + j] : In(0);
-                    t2[j] = kj[kb] < a.tk ? V[kj[kb] * D + kc * 32 + g * 8 + j] : In(0);
+                    t1[j] = kj[kb] < a.tk ? t1[j] : In(0);
+                    t2[j] = kj[kb] < a.tk ? t2[j] : In(0);
                 }
                 make_frag<P>(kf[kb][kc], t1);
                 make_frag<P>(vf[kb][kc], t2);
@@ -801,8 +813,10 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         ColT::load(pdt, DT + q0_, a.tqp, tplane);                                            \
         if (tid < BN) {                                                                      \
             const int64_t xq_ = q0_ + tid;                                                   \
-            pm = xq_ < a.tq ? RM[xq_] * (float)LOG2E + __log2f(RS[xq_]) : 0.f;               \
-            pdl = xq_ < a.tq ? DL[xq_] : 0.f;                                                \
+            const int64_t xc_ = xq_ < a.tq ? xq_ : a.tq - 1;                                  \
+            const float rm_ = RM[xc_], rs_ = RS[xc_], dl_ = DL[xc_];                         \
+            pm = xq_ < a.tq ? rm_ * (float)LOG2E + __log2f(rs_) : 0.f;                       \
+            pdl = xq_ < a.tq ? dl_ : 0.f;                                                    \
         }                                                                                    \
     } while (0)
         if (PF && qt_begin < qt_end) EF_FETCH(qt_begin);
@@ -1007,17 +1021,26 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
             for (int kc = 0; kc < KC; kc++) {
                 In t1[8], t2[8];
 #pragma unroll
+                for (int j = 0; j < 8; j++) {   // unconditional loads (row clamped), then 0 past tq
+                    const int64_t xr = x[qb] < a.tq ? x[qb] : a.tq - 1;
+                    t1[j] = Q[xr * D + kc * 32 + g * 8 + j];
+                    t2[j] = DO[xr * D + kc * 32 + g * 8 + j];
+                }
+#pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    t1[j] = x[qb] < a.tq ? Q[x[qb] * D + kc * 32 + g * 8 + j] : In(0);
-                    t2[j] = x[qb] < a.tq ? DO[x[qb] * D + kc * 32 + g * 8 + j] : In(0);
+                    t1[j] = x[qb] < a.tq ? t1[j] : In(0);
+                    t2[j] = x[qb] < a.tq ? t2[j] : In(0);
                 }
                 make_frag<P>(qf[qb][kc], t1);
                 make_frag<P>(df[qb][kc], t2);
             }
             // m log2(e) + log2(l): P = exp2(scale s log2(e) - mrow).
-            mrow[qb] = x[qb] < a.tq ? (S)a.row_max[b * a.tq + x[qb]] * (S)LOG2E + (S)__log2f(a.row_sum[b * a.tq + x[qb]])
-                                    : S(0);
-            drow[qb] = x[qb] < a.tq ? (S)a.delta[b * a.tq + x[qb]] : S(0);
+            {
+                const int64_t xr = b * a.tq + (x[qb] < a.tq ? x[qb] : a.tq - 1);
+                const float rm = a.row_max[xr], rsum = a.row_sum[xr], dl = a.delta[xr];
+                mrow[qb] = x[qb] < a.tq ? (S)rm * (S)LOG2E + (S)__log2f(rsum) : S(0);
+                drow[qb] = x[qb] < a.tq ? (S)dl : S(0);
+            }
         }
         int64_t first, last;
         group_range(sp, red_lo, red_hi, first, last);
