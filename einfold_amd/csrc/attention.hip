@@ -803,21 +803,22 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         typedef TileIO<E, PL, D, BN> ColT;   // [feature][query]
         constexpr bool PF = 2 * (RowT::N + ColT::N) <= 16;
         u32x4 pq[RowT::N], pqt[ColT::N], pd[RowT::N], pdt[ColT::N];
-        float pm = 0.f, pdl = 0.f;
+        float prm = 0.f, prs = 1.f, pdl = 0.f;
 #define EF_FETCH(T_)                                                                         \
     do {                                                                                     \
         const int64_t q0_ = (T_) * BN;                                                       \
+        /* row statistics first: loads complete in order, so using them */                  \
+        /* later waits for nothing behind them */                                            \
+        if (tid < BN) {                                                                      \
+            const int64_t xc_ = q0_ + tid < a.tq ? q0_ + tid : a.tq - 1;                     \
+            prm = RM[xc_];                                                                   \
+            prs = RS[xc_];                                                                   \
+            pdl = DL[xc_];                                                                   \
+        }                                                                                    \
         RowT::load(pq, QR + q0_ * D, D, rplane);                                             \
         ColT::load(pqt, QT + q0_, a.tqp, tplane);                                            \
         RowT::load(pd, DR + q0_ * D, D, rplane);                                             \
         ColT::load(pdt, DT + q0_, a.tqp, tplane);                                            \
-        if (tid < BN) {                                                                      \
-            const int64_t xq_ = q0_ + tid;                                                   \
-            const int64_t xc_ = xq_ < a.tq ? xq_ : a.tq - 1;                                  \
-            const float rm_ = RM[xc_], rs_ = RS[xc_], dl_ = DL[xc_];                         \
-            pm = xq_ < a.tq ? rm_ * (float)LOG2E + __log2f(rs_) : 0.f;                       \
-            pdl = xq_ < a.tq ? dl_ : 0.f;                                                    \
-        }                                                                                    \
     } while (0)
         if (PF && qt_begin < qt_end) EF_FETCH(qt_begin);
         for (int64_t qt = qt_begin; qt < qt_end; qt++) {
@@ -827,19 +828,23 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
             ColT::store(qtr, TP, TPLANE, pqt);
             RowT::store(dos, RP, RPLANE, pd);
             ColT::store(dotr, TP, TPLANE, pdt);
-            bool any = false, full = true;
-            if (tid < BN) {
-                const int64_t xq = q0 + tid;
+            // Every wave votes over the tile's 64 queries (one per lane), so
+            // all agree without a barrier: does any see a key of the block,
+            // does each see all of them.
+            bool any_tile, full_tile;
+            {
+                const int64_t xq = q0 + lane;
                 const Span sp = query_span(a.mask, xq);
-                slo[tid] = sp.lo;
-                shi[tid] = sp.hi;
-                sm[tid] = pm;
-                sd[tid] = pdl;
-                any = sp.lo < sp.hi && sp.lo < kvalid && sp.hi > k0;
-                full = xq >= a.tq || (sp.lo <= k0 && sp.hi >= kvalid);
+                any_tile = __any(sp.lo < sp.hi && sp.lo < kvalid && sp.hi > k0);
+                full_tile = __all(xq >= a.tq || (sp.lo <= k0 && sp.hi >= kvalid));
+                if (tid < BN) {
+                    slo[tid] = sp.lo;
+                    shi[tid] = sp.hi;
+                    sm[tid] = xq < a.tq ? prm * (float)LOG2E + __log2f(prs) : 0.f;
+                    sd[tid] = xq < a.tq ? pdl : 0.f;
+                }
             }
-            const bool any_tile = __syncthreads_or(any);
-            const bool full_tile = __syncthreads_and(full);
+            __syncthreads();
             if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
             if (any_tile) {
             // The tile's queries in two 32-query halves (hc): S = Q K^T and
@@ -1080,7 +1085,9 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
             bool full = true;
 #pragma unroll
             for (int qb = 0; qb < 2; qb++) full &= x[qb] >= a.tq || (sp[qb].lo <= j0 && sp[qb].hi >= jvalid);
-            const bool full_tile = __syncthreads_and(full);
+            // Each wave picks the mask-free path for its own queries.
+            const bool full_tile = __all(full);
+            __syncthreads();
             if (PF && j0 + BN < last) EF_FETCH(j0 + BN);
             // The tile's keys in two 32-key halves: S^T = K Q^T and
             // dP^T = V dO^T, then dS^T, then dQ^T += K^T dS^T.
