@@ -174,13 +174,32 @@ __global__ void zero_counters_kernel(Counter *c) {
     if (threadIdx.x < 4) { c[threadIdx.x].next = 0; c[threadIdx.x].done = 0; }
 }
 
-__device__ __forceinline__ int take_item(Counter *ctr, int *slot) {
-    if (threadIdx.x == 0) *slot = (int)atomicAdd(&ctr->next, 1u);
-    __syncthreads();
-    const int item = *slot;
-    __syncthreads();
-    return item;
-}
+// A workgroup's items: its first is blockIdx.x, the rest come from the
+// counter (offset by the grid), each fetched one item ahead so the
+// atomic's round trip overlaps the current item's work.
+struct ItemQueue {
+    Counter *ctr;
+    int *slot;
+    unsigned ahead;   // thread 0: the item after the one in slot
+    __device__ __forceinline__ void start(Counter *c, int *s) {
+        ctr = c;
+        slot = s;
+        if (threadIdx.x == 0) {
+            *slot = (int)blockIdx.x;
+            ahead = gridDim.x + atomicAdd(&ctr->next, 1u);
+        }
+    }
+    __device__ __forceinline__ int next() {
+        __syncthreads();
+        const int item = *slot;
+        __syncthreads();   // all have read slot before thread 0 writes it again
+        if (threadIdx.x == 0) {
+            *slot = (int)ahead;
+            ahead = gridDim.x + atomicAdd(&ctr->next, 1u);
+        }
+        return item;
+    }
+};
 
 // Copies `rows` rows of `cols` elements (16-byte chunks) of every plane
 // from global (row stride gstride, plane stride gplane) into an LDS tile
@@ -233,32 +252,30 @@ template <class E, int PLN, int R, int C> struct TileIO {
     }
 };
 
-// The union of the spans of a workgroup's queries (two per lane).
-template <class SP>
-__device__ __forceinline__ void group_range(const SP (&sp)[2], int64_t *red_lo, int64_t *red_hi, int64_t &first,
-                                            int64_t &last) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// The union [first, last) of the spans of queries q0 .. q0 + BW - 1,
+// computed by each wave on its own (BW / 64 queries per lane, a wave-wide
+// min / max): no LDS and no barrier.  Empty: first > last.
+__device__ __forceinline__ void block_range(const Mask &m, int64_t q0, int64_t &first, int64_t &last) {
+    const int lane = threadIdx.x & 63;
     int64_t lo = INT64_MAX, hi = 0;
 #pragma unroll
-    for (int qb = 0; qb < 2; qb++)
-        if (sp[qb].lo < sp[qb].hi) {
-            lo = sp[qb].lo < lo ? sp[qb].lo : lo;
-            hi = sp[qb].hi > hi ? sp[qb].hi : hi;
+    for (int h = 0; h < BW / 64; h++) {
+        const Span sp = query_span(m, q0 + h * 64 + lane);
+        if (sp.lo < sp.hi) {
+            lo = sp.lo < lo ? sp.lo : lo;
+            hi = sp.hi > hi ? sp.hi : hi;
         }
-    for (int o = 32; o >= 1; o >>= 1) {
-        const int64_t l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
-        lo = l2 < lo ? l2 : lo;
-        hi = h2 > hi ? h2 : hi;
     }
-    if (lane == 0) { red_lo[w] = lo; red_hi[w] = hi; }
-    __syncthreads();
-    first = red_lo[0];
-    last = red_hi[0];
-    for (int i = 1; i < WAVES; i++) {
-        first = red_lo[i] < first ? red_lo[i] : first;
-        last = red_hi[i] > last ? red_hi[i] : last;
+    // Spans lie in [0, tk], tk < 2^31: reduce in 32 bits.
+    int l32 = lo == INT64_MAX ? INT32_MAX : (int)lo, h32 = (int)hi;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int l2 = __shfl_xor(l32, o), h2 = __shfl_xor(h32, o);
+        l32 = l2 < l32 ? l2 : l32;
+        h32 = h2 > h32 ? h2 : h32;
     }
-    __syncthreads();
+    first = l32 == INT32_MAX ? INT64_MAX : l32;
+    last = h32;
 }
 
 // ---------------------------------------------------------------------
@@ -310,7 +327,6 @@ void attn_fwd_kernel(FwdArgs a) {
     typedef FwdTiles<P, D> F;
     constexpr int KC = D / 32, DB = D / 16, NP = Parts<P>::n;
     __shared__ __attribute__((aligned(1024))) E lds[2 * F::BUF];
-    __shared__ int64_t red_lo[WAVES], red_hi[WAVES];
     __shared__ int flags[2][WAVES];   // per buffer, per wave: bit 0 any, bit 1 full
     __shared__ int slot;
 
@@ -340,8 +356,10 @@ void attn_fwd_kernel(FwdArgs a) {
             }
         }
     }
+    ItemQueue queue;
+    queue.start(a.ctr, &slot);
     for (;;) {
-        const int item = take_item(a.ctr, &slot);
+        const int item = queue.next();
         if (item >= a.items) break;
         const int part = item % a.parts;
         const int64_t blk = item / a.parts;
@@ -376,7 +394,7 @@ void attn_fwd_kernel(FwdArgs a) {
             }
         }
         int64_t first, last;
-        group_range(sp, red_lo, red_hi, first, last);
+        block_range(a.mask, q0, first, last);
         int64_t jbeg = first < last ? first - first % BN : last;
         if (a.parts == 2) {
             // Part 0 takes the first half of the key tiles, part 1 the rest.
@@ -739,8 +757,10 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     const int64_t rplane = a.batches * a.tqp * D, tplane = a.batches * D * a.tqp;
     const S scale2 = (S)a.scale * (S)LOG2E;
 
+    ItemQueue queue;
+    queue.start(a.ctr, &slot);
     for (;;) {
-        const int item = take_item(a.ctr, &slot);
+        const int item = queue.next();
         if (item >= a.items) break;
         const int part = item % a.parts;
         const int64_t blk = item / a.parts;
@@ -991,7 +1011,6 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     __shared__ __attribute__((aligned(16))) E ks[PL * RPLANE];
     __shared__ __attribute__((aligned(16))) E ktr[PL * TPLANE];
     __shared__ __attribute__((aligned(16))) E vs[PL * RPLANE];
-    __shared__ int64_t red_lo[WAVES], red_hi[WAVES];
     __shared__ int slot;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
@@ -1000,8 +1019,10 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     const int64_t rplane = a.batches * a.tkp * D, tplane = a.batches * D * a.tkp;
     const S scale2 = (S)a.scale * (S)LOG2E;
 
+    ItemQueue queue;
+    queue.start(a.ctr, &slot);
     for (;;) {
-        const int item = take_item(a.ctr, &slot);
+        const int item = queue.next();
         if (item >= a.items) break;
         const int part = item % a.parts;
         const int64_t blk = item / a.parts;
@@ -1048,7 +1069,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
             }
         }
         int64_t first, last;
-        group_range(sp, red_lo, red_hi, first, last);
+        block_range(a.mask, q0, first, last);
         int64_t jbeg = first < last ? first - first % BN : last;
         if (a.parts == 2) {
             const int64_t cut = jbeg + ((last - jbeg + 2 * BN - 1) / (2 * BN)) * BN;
