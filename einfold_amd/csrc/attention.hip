@@ -174,29 +174,26 @@ __global__ void zero_counters_kernel(Counter *c) {
     if (threadIdx.x < 4) { c[threadIdx.x].next = 0; c[threadIdx.x].done = 0; }
 }
 
-// A workgroup's items: its first is blockIdx.x, the rest come from the
-// counter (offset by the grid), each fetched one item ahead so the
-// atomic's round trip overlaps the current item's work.
+// A workgroup's items: its first is blockIdx.x, the rest (when the grid
+// is smaller than the item count) come from the counter, offset by the
+// grid.  A grid of one workgroup per item never touches the counter: the
+// hardware hands out workgroups in index order, heaviest items first.
 struct ItemQueue {
     Counter *ctr;
     int *slot;
-    unsigned ahead;   // thread 0: the item after the one in slot
-    __device__ __forceinline__ void start(Counter *c, int *s) {
+    int items;
+    __device__ __forceinline__ void start(Counter *c, int *s, int n) {
         ctr = c;
         slot = s;
-        if (threadIdx.x == 0) {
-            *slot = (int)blockIdx.x;
-            ahead = gridDim.x + atomicAdd(&ctr->next, 1u);
-        }
+        items = n;
+        if (threadIdx.x == 0) *slot = (int)blockIdx.x;
     }
     __device__ __forceinline__ int next() {
         __syncthreads();
         const int item = *slot;
         __syncthreads();   // all have read slot before thread 0 writes it again
-        if (threadIdx.x == 0) {
-            *slot = (int)ahead;
-            ahead = gridDim.x + atomicAdd(&ctr->next, 1u);
-        }
+        if (threadIdx.x == 0)
+            *slot = (int)gridDim.x >= items ? items : (int)(gridDim.x + atomicAdd(&ctr->next, 1u));
         return item;
     }
 };
@@ -357,7 +354,7 @@ void attn_fwd_kernel(FwdArgs a) {
         }
     }
     ItemQueue queue;
-    queue.start(a.ctr, &slot);
+    queue.start(a.ctr, &slot, a.items);
     for (;;) {
         const int item = queue.next();
         if (item >= a.items) break;
@@ -758,7 +755,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     const S scale2 = (S)a.scale * (S)LOG2E;
 
     ItemQueue queue;
-    queue.start(a.ctr, &slot);
+    queue.start(a.ctr, &slot, a.items);
     for (;;) {
         const int item = queue.next();
         if (item >= a.items) break;
@@ -1020,7 +1017,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     const S scale2 = (S)a.scale * (S)LOG2E;
 
     ItemQueue queue;
-    queue.start(a.ctr, &slot);
+    queue.start(a.ctr, &slot, a.items);
     for (;;) {
         const int item = queue.next();
         if (item >= a.items) break;
