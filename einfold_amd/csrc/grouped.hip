@@ -28,7 +28,7 @@ namespace ef {
 
 namespace {
 
-constexpr int KS = 32;          // k per step
+constexpr int KS = 64;          // k per step
 constexpr int PADT = 128;       // rows and columns padded to this
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
@@ -69,7 +69,10 @@ __global__ __launch_bounds__(256) void grouped_prep_kernel(const GemmDesc *descs
         for (int i = tid; i < 64 * 64; i += 256) {
             const int r = i / 64, kk = i % 64;
             const int64_t row = r0 + r, col = k0 + kk;
-            const float v = (row < d.m && col < d.k) ? d.a[row * d.k + col] : 0.f;
+            // Unconditional load from a clamped cell, then 0 outside: the
+            // loads issue together instead of one branch and wait each.
+            const float a = d.a[(row < d.m ? row : d.m - 1) * d.k + (col < d.k ? col : d.k - 1)];
+            const float v = (row < d.m && col < d.k) ? a : 0.f;
             if (row < d.mp && col < d.kp) {
                 if constexpr (P == PREC_BF16X3) {
                     __bf16 h, l;
@@ -88,7 +91,8 @@ __global__ __launch_bounds__(256) void grouped_prep_kernel(const GemmDesc *descs
         for (int i = tid; i < 64 * 64; i += 256) {
             const int kk = i / 64, r = i % 64;
             const int64_t kx = k0 + kk, col = r0 + r;
-            t[kk][r] = (kx < d.k && col < d.n) ? d.b[kx * d.n + col] : 0.f;
+            const float b = d.b[(kx < d.k ? kx : d.k - 1) * d.n + (col < d.n ? col : d.n - 1)];
+            t[kk][r] = (kx < d.k && col < d.n) ? b : 0.f;
         }
         __syncthreads();
         for (int i = tid; i < 64 * 64; i += 256) {
@@ -130,7 +134,6 @@ __global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *de
     constexpr int WM = BM / 2, WN = BN / 2, IM = WM / 16, JN = WN / 16;
     constexpr int STEPS = KS / 32;   // 32-wide fragment chunks per k step
     __shared__ __attribute__((aligned(16))) E lds[2 * BUF];
-    __shared__ int slot;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
     const int wr = w >> 1, wc = w & 1;
     // This thread's chunks: (plane, row, col) of the A and B tiles.
@@ -144,12 +147,10 @@ __global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *de
         crow_[i] = (kq % (rows * per_row)) / per_row;
         ccol[i] = (kq % per_row) * CE;
     }
-    for (;;) {
-        if (tid == 0) slot = (int)atomicAdd(&ctr->next, 1u);
-        __syncthreads();
-        const int it = slot;
-        __syncthreads();
-        if (it >= n_items) break;
+    (void)ctr;
+    // Items are sorted heaviest first; workgroup g takes g, g + grid, ...
+    // (the hardware starts workgroups in index order as CUs free up).
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
         const Item item = items[it];
         const GemmDesc d = descs[item.problem];
         const int64_t m0 = (int64_t)item.tm * BM, n0 = (int64_t)item.tn * BN;
@@ -172,9 +173,11 @@ __global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *de
         E *dst_ = (BUFP) + (i_ < AN ? cpl[i_] * APLANE : PL * APLANE + cpl[i_] * BPLANE);             \
         *reinterpret_cast<u32x4 *>(dst_ + crow_[i_] * PITCH + ccol[i_]) = pre[i_];                    \
     }
-        const int64_t nk = d.kp / KS;
-        EF_GLOAD(0);
-        EF_LSTORE(lds);
+        const int64_t nk = d.kp / KS;   // 0 for k = 0: no loads, C = 0
+        if (nk > 0) {
+            EF_GLOAD(0);
+            EF_LSTORE(lds);
+        }
         __syncthreads();
         for (int64_t ks = 0; ks < nk; ks++) {
             const int cur = (int)(ks & 1);
@@ -269,7 +272,7 @@ ef_status build(GroupedPlan &p, hipStream_t s) {
         d.m = (int64_t)q.m; d.n = (int64_t)q.n; d.k = (int64_t)q.k;
         d.mp = (d.m + PADT - 1) / PADT * PADT;
         d.np = (d.n + PADT - 1) / PADT * PADT;
-        d.kp = (d.k + 63) / 64 * 64;
+        d.kp = (d.k + KS - 1) / KS * KS;
         d.a_off = off; off += (int64_t)planes * d.mp * d.kp;
         d.b_off = off; off += (int64_t)planes * d.np * d.kp;
         descs.push_back(d);
@@ -311,7 +314,6 @@ ef_status launch_steps(const GroupedPlan &p, hipStream_t s) {
     typedef typename Elem<P>::T E;
     if (p.n_items == 0) return EF_OK;
     hipLaunchKernelGGL(grouped_prep_kernel<P>, dim3((unsigned)p.n_tasks), dim3(256), 0, s, p.descs, p.tasks, (E *)p.ops);
-    hipLaunchKernelGGL(grouped_zero_kernel, dim3(1), dim3(64), 0, s, p.ctr);
     if (p.config == 0)
         hipLaunchKernelGGL((grouped_gemm_kernel<P, 128, 128>), dim3((unsigned)p.grid), dim3(256), 0, s, p.descs,
                            p.items, p.n_items, (const E *)p.ops, p.ctr);

@@ -70,3 +70,19 @@ def test_autotuner_picks_a_tile(cuda):
     plan.run()
     torch.cuda.synchronize()
     _check(problems, TOL["fp32"])
+
+
+def test_empty_dimensions(cuda):
+    """k = 0 gives a zero product (nothing is read); m = 0 or n = 0 writes
+    nothing; the other products in the same launch are unaffected."""
+    shapes = [(5, 7, 0), (0, 3, 4), (3, 0, 4), (64, 64, 64), (70, 9, 0)]
+    problems = _problems(shapes, cuda, seed=2)
+    plan = device.GroupedPlan(problems, "bf16x3")
+    plan.run()
+    torch.cuda.synchronize()
+    for (m, n, k), (a, b, c) in zip(shapes, problems):
+        if k == 0:
+            assert torch.equal(c, torch.zeros_like(c))
+        elif m and n:
+            _check([(a, b, c)], TOL["bf16x3"])
+    plan.close()
