@@ -23,20 +23,37 @@ constexpr int THREADS = 256;
 constexpr int ROUNDS = 16;                    // keys per thread per tile
 constexpr int TILE = THREADS * ROUNDS;        // 4096 keys per tile
 
+// OR and AND of all words: one tile of TILE words per block, all of a
+// thread's loads issued together (clamped rows: OR and AND don't change
+// when a word is counted twice), one atomic pair per block.
 __global__ __launch_bounds__(256) void or_and_kernel(const uint64_t *w, int64_t n,
                                                      unsigned long long *any, unsigned long long *all) {
+    __shared__ uint64_t red_o[4], red_a[4];
+    const int64_t base = (int64_t)blockIdx.x * TILE;
+    uint64_t v[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const int64_t i = base + r * THREADS + threadIdx.x;
+        v[r] = w[i < n ? i : n - 1];
+    }
     uint64_t o = 0, a = ~0ULL;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        o |= w[i];
-        a &= w[i];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        o |= v[r];
+        a &= v[r];
     }
     for (int off = 32; off >= 1; off >>= 1) {
         o |= __shfl_xor(o, off);
         a &= __shfl_xor(a, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicOr(any, (unsigned long long)o);
-        atomicAnd(all, (unsigned long long)a);
+        red_o[threadIdx.x >> 6] = o;
+        red_a[threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicOr(any, (unsigned long long)(red_o[0] | red_o[1] | red_o[2] | red_o[3]));
+        atomicAnd(all, (unsigned long long)(red_a[0] & red_a[1] & red_a[2] & red_a[3]));
     }
 }
 
@@ -47,10 +64,17 @@ __global__ __launch_bounds__(256) void histogram_kernel(const uint64_t *__restri
     h[tid] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * TILE;
+    // All of the tile's loads first, from clamped rows (unconditional
+    // loads issue together instead of one branch and wait each).
+    uint64_t v[ROUNDS];
+#pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
         const int64_t i = base + r * THREADS + tid;
-        if (i < n) atomicAdd(&h[(w[i] >> shift) & 0xff], 1u);
+        v[r] = w[i < n ? i : n - 1];
     }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++)
+        if (base + r * THREADS + tid < n) atomicAdd(&h[(v[r] >> shift) & 0xff], 1u);
     __syncthreads();
     counts[(int64_t)tid * tiles + blockIdx.x] = h[tid];
 }
@@ -68,11 +92,22 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint64_t *__restrict
     run[tid] = offsets[(int64_t)tid * tiles + blockIdx.x];
     const int64_t base = (int64_t)blockIdx.x * TILE;
     const uint64_t below = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    // The tile's words and rows loaded up front, from clamped rows.
+    uint64_t words[ROUNDS];
+    uint32_t rws[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const int64_t i = base + r * THREADS + tid;
+        const int64_t ic = i < n ? i : n - 1;
+        words[r] = w_in[ic];
+        rws[r] = rows_in ? rows_in[ic] : (uint32_t)i;
+    }
+#pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
         const int64_t i = base + r * THREADS + tid;
         const bool valid = i < n;
-        const uint64_t word = valid ? w_in[i] : 0;
-        const uint32_t row = valid ? (rows_in ? rows_in[i] : (uint32_t)i) : 0;
+        const uint64_t word = valid ? words[r] : 0;
+        const uint32_t row = valid ? rws[r] : 0;
         const uint32_t d = (uint32_t)((word >> shift) & 0xff);
         // Lanes of this wave with the same byte (and a row).
         uint64_t peers = __ballot(valid);
@@ -230,7 +265,7 @@ ef_status radix_passes(const uint64_t *words, const uint32_t *rows_in, size_t n,
                        const uint64_t *&w_cur, const uint32_t *&r_cur, hipStream_t s) {
     const unsigned long long init[2] = {0ULL, ~0ULL};
     EF_HIP(hipMemcpyAsync(x.bits, init, sizeof init, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(or_and_kernel, dim3(256), dim3(256), 0, s, words, (int64_t)n, x.bits, x.bits + 1);
+    hipLaunchKernelGGL(or_and_kernel, dim3((unsigned)x.tiles), dim3(256), 0, s, words, (int64_t)n, x.bits, x.bits + 1);
     unsigned long long host_bits[2];
     EF_HIP(hipMemcpyAsync(host_bits, x.bits, sizeof host_bits, hipMemcpyDeviceToHost, s));
     EF_HIP(hipStreamSynchronize(s));
