@@ -3,30 +3,29 @@
 // (word, row) pairs that skips the bytes no word varies in, then each
 // row's rank among the distinct words and each rank's first row.
 //
-// One read of the words builds all eight byte histograms (and clears the
-// look-back words); a small launch adds up the workgroups' histograms, from
-// which the host learns which bytes vary.  Then each varying byte is one
-// single-pass scatter (the decoupled look-back of Merrill and Garland,
-// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016, used
-// for a radix pass as in Adinets and Merrill's "Onesweep", 2022).  Each
-// tile of 4096 pairs
+// One read of the words builds all eight byte histograms; a small launch
+// adds up the workgroups' histograms, from which the host learns which
+// bytes vary.  Each varying byte is then one pass of three launches
+// (reduce, then scan, then scatter): each tile of 4096 pairs counts its
+// values of the byte; a scan over the tiles of each value gives every tile
+// the start of its run of each value; and each tile
 //   1. ranks its pairs among the tile's pairs with the same byte, in row
 //      order: a wave takes 1024 consecutive rows, 64 at a time, finds the
 //      lanes that share its byte with eight ballots and counts them in a
 //      histogram of its own in LDS (no barrier inside the wave);
-//   2. publishes its 256 counts, then walks back over the tiles before it
-//      for the counts of each byte value in them: a tile's word holds
-//      either its own count or the count of it and every tile before it,
-//      and the walk stops at the first of the latter;
-//   3. stages its pairs in LDS in sorted order and writes each byte value's
-//      run to its place, so that neighbouring lanes write neighbouring
-//      words.
-// Tiles take their numbers from a counter in the order they start, so every
-// tile a walk waits for is already running.  Each wait is bounded: past the
-// limit the kernel sets an error flag (the host reports it) and goes on.
-// A last launch counts the distinct words of the sorted pairs the same way
-// (one count per tile) and writes each row's rank and each rank's first
-// row.  All of it is integer arithmetic: exact and deterministic.
+//   2. stages its pairs in LDS in sorted order and writes each value's run
+//      to its place, so that neighbouring lanes write neighbouring words.
+// (A single-pass scatter with a decoupled look-back across tiles, as in
+// Adinets and Merrill's "Onesweep", 2022, measured slower here: every tile
+// waits for the slowest tile before it.)
+// The ranks come the same way: each tile of sorted positions counts the
+// positions that start a rank (word differs from the one before) and,
+// for large inputs, its rows in each of up to 256 buckets of rows; a scan
+// gives each tile its first rank and its runs of (row, rank) pairs; one
+// workgroup per bucket then gathers its pairs in an LDS window and writes
+// the bucket's ranks in whole lines (4-byte stores to random rows would
+// cost HBM a 32-byte sector each).  All of it is integer arithmetic: exact
+// and deterministic.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -41,109 +40,17 @@ constexpr int WAVES = THREADS / 64;
 constexpr int KPT = 16;                     // pairs per thread per tile
 constexpr int TILE = THREADS * KPT;         // 4096 pairs per tile
 constexpr int WAVE_PAIRS = 64 * KPT;        // consecutive rows of one wave
-constexpr int HIST_BLOCKS = 512;            // workgroups of the histogram read
-constexpr unsigned SPIN_LIMIT = 1u << 22;   // polls of one look-back word
-constexpr unsigned ASSIGN_PASS = 8;         // look-back tag of the ranking launch
 
 constexpr int HIST_CHUNKS = 64;             // workgroups per byte adding the histograms up
 
-// Small device words of a call: the tile counters of the eight byte passes
-// and of the ranking launch, the error flag and the number of ranks.
+// Small device words of a call (the number of ranks is word S_RANKS).
 constexpr int SMALL_WORDS = 64;
-constexpr int S_ERR = 9, S_RANKS = 10;
-constexpr unsigned ERR_WAIT = 1;   // bit of the error flag
-
-// Look-back words are (tag << 32) | count.  In pass p, tag 2p+1 carries a
-// tile's own count and 2p+2 the count of the tile and every tile before
-// it; 0 (cleared) or another pass's tag means "not written yet".
-__device__ __forceinline__ unsigned long long lb_word(unsigned tag, uint32_t count) {
-    return ((unsigned long long)tag << 32) | count;
-}
-
-__device__ __forceinline__ void lb_store(unsigned long long *p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ unsigned long long lb_load(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sum of the counts of tiles 0 .. tile-1 in column `col` (256 words per
-// tile) of pass `pass`, one thread per column.  Each round reads the words
-// of LB_WINDOW tiles at once (nearest first; tile 0 stands in for tiles
-// below it, which the walk never reaches: tile 0's word is the last one it
-// can need), adds the own-counts down to the first word that is not
-// written yet and stops at the first word that covers every earlier tile.
-constexpr int LB_WINDOW = 16;
-
-__device__ uint32_t look_back(const unsigned long long *words, int64_t tile, int col, unsigned pass,
-                              unsigned *err) {
-    const unsigned own = 2 * pass + 1, all = 2 * pass + 2;
-    uint32_t sum = 0;
-    unsigned spins = 0;
-    for (int64_t j = tile - 1; j >= 0;) {
-        unsigned long long v[LB_WINDOW];
-#pragma unroll
-        for (int w = 0; w < LB_WINDOW; w++) v[w] = lb_load(words + (j - w > 0 ? j - w : 0) * 256 + col);
-        int used = 0;
-        bool done = false, blocked = false;
-#pragma unroll
-        for (int w = 0; w < LB_WINDOW; w++) {
-            if (done || blocked) continue;
-            const unsigned tag = (unsigned)(v[w] >> 32);
-            if (tag == all || tag == own) {
-                sum += (uint32_t)v[w];
-                used++;
-                done = tag == all;
-            } else {
-                blocked = true;
-            }
-        }
-        if (done) return sum;
-        j -= used;
-        if (used == 0) {
-            if (++spins > SPIN_LIMIT) {
-                atomicOr(err, ERR_WAIT);
-                return sum;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    return sum;
-}
+constexpr int S_RANKS = 10;
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
     return v;
-}
-
-// The same walk for one column (column 0) by a whole wave: lane l reads
-// tile j - l, 64 tiles a round.  Every lane returns the sum.
-__device__ uint32_t look_back_wave(const unsigned long long *words, int64_t tile, unsigned pass, unsigned *err,
-                                   int lane) {
-    const unsigned own = 2 * pass + 1, all = 2 * pass + 2;
-    uint32_t sum = 0;
-    unsigned spins = 0;
-    for (int64_t j = tile - 1; j >= 0;) {
-        const unsigned long long v = lb_load(words + (j - lane > 0 ? j - lane : 0) * 256);
-        const unsigned tag = (unsigned)(v >> 32);
-        const uint64_t covers = __ballot(tag == all);
-        const uint64_t waiting = __ballot(tag != all && tag != own);
-        const int first_cover = covers ? __builtin_ctzll(covers) : 64;
-        const int first_wait = waiting ? __builtin_ctzll(waiting) : 64;
-        if (first_cover < first_wait) return sum + wave_sum(lane <= first_cover ? (uint32_t)v : 0u);
-        sum += wave_sum(lane < first_wait ? (uint32_t)v : 0u);
-        j -= first_wait;
-        if (first_wait == 0) {
-            if (++spins > SPIN_LIMIT) {
-                if (lane == 0) atomicOr(err, ERR_WAIT);
-                return sum;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    return sum;
 }
 
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v, int lane) {
@@ -158,51 +65,47 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v, int lane) {
 struct HistArgs {
     const uint64_t *w;
     int64_t n, tiles;
-    uint32_t *partial;          // [gridDim.x][8][256]
-    unsigned long long *lb;     // look-back words to clear
-    int64_t lb_n;
+    uint32_t *partial;          // [tiles][8][256]
     unsigned *small;            // SMALL_WORDS words to clear
     uint32_t *counts;           // [8][256], cleared here, summed by the next launch
 };
 
-// The eight byte histograms of each workgroup's tiles, in LDS.
+// The eight byte histograms of tile blockIdx.x, in LDS.  A byte's
+// histograms are also the tile counts of the first pass, which reads the
+// words in this order.
 __global__ __launch_bounds__(THREADS) void byte_histograms_kernel(HistArgs a) {
     __shared__ uint32_t h[8][256];
     const int tid = threadIdx.x, lane = tid & 63;
 #pragma unroll
     for (int b = 0; b < 8; b++) h[b][tid] = 0;
-    for (int64_t i = (int64_t)blockIdx.x * THREADS + tid; i < a.lb_n; i += (int64_t)gridDim.x * THREADS)
-        a.lb[i] = 0;
     if (blockIdx.x == 0) {
         if (tid < SMALL_WORDS) a.small[tid] = 0;
 #pragma unroll
         for (int b = 0; b < 8; b++) a.counts[b * 256 + tid] = 0;
     }
+    const int64_t base = (int64_t)blockIdx.x * TILE;
+    uint64_t v[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int64_t i = base + r * THREADS + tid;
+        v[r] = a.w[i < a.n ? i : a.n - 1];
+    }
     __syncthreads();
-    for (int64_t t = blockIdx.x; t < a.tiles; t += gridDim.x) {
-        const int64_t base = t * TILE;
-        uint64_t v[KPT];
 #pragma unroll
-        for (int r = 0; r < KPT; r++) {
-            const int64_t i = base + r * THREADS + tid;
-            v[r] = a.w[i < a.n ? i : a.n - 1];
-        }
+    for (int r = 0; r < KPT; r++) {
+        const bool valid = base + r * THREADS + tid < a.n;
+        const uint64_t live = __ballot(valid);
 #pragma unroll
-        for (int r = 0; r < KPT; r++) {
-            const bool valid = base + r * THREADS + tid < a.n;
-            const uint64_t live = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const uint32_t d = (uint32_t)(v[r] >> (8 * b)) & 0xff;
-                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-                // One add for the wave when every live lane has the same
-                // byte (live lanes are a prefix of the wave, so lane 0 is
-                // one of them).
-                if (__ballot(valid && d == d0) == live) {
-                    if (lane == 0 && live) atomicAdd(&h[b][d0], (uint32_t)__popcll(live));
-                } else if (valid) {
-                    atomicAdd(&h[b][d], 1u);
-                }
+        for (int b = 0; b < 8; b++) {
+            const uint32_t d = (uint32_t)(v[r] >> (8 * b)) & 0xff;
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            // One add for the wave when every live lane has the same byte
+            // (live lanes are a prefix of the wave, so lane 0 is one of
+            // them).
+            if (__ballot(valid && d == d0) == live) {
+                if (lane == 0 && live) atomicAdd(&h[b][d0], (uint32_t)__popcll(live));
+            } else if (valid) {
+                atomicAdd(&h[b][d], 1u);
             }
         }
     }
@@ -218,9 +121,96 @@ __global__ __launch_bounds__(THREADS) void byte_histograms_kernel(HistArgs a) {
 __global__ __launch_bounds__(THREADS) void sum_histograms_kernel(const uint32_t *partial, int blocks,
                                                                  uint32_t *counts) {
     const int tid = threadIdx.x, c = blockIdx.x, b = blockIdx.y;
+    const uint32_t *col = partial + b * 256 + tid;
     uint32_t s = 0;
-    for (int k = c; k < blocks; k += HIST_CHUNKS) s += partial[(int64_t)k * 2048 + b * 256 + tid];
+    int k = c;
+    for (; k + 7 * HIST_CHUNKS < blocks; k += 8 * HIST_CHUNKS) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = col[(int64_t)(k + j * HIST_CHUNKS) * 2048];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s += v[j];
+    }
+    for (; k < blocks; k += HIST_CHUNKS) s += col[(int64_t)k * 2048];
     if (s) atomicAdd(&counts[b * 256 + tid], s);
+}
+
+// Tile t's count of each value of the byte at `shift`: tile_counts[d][t].
+template <class W>
+__global__ __launch_bounds__(THREADS) void tile_counts_kernel(const W *__restrict__ w, int64_t n, int shift,
+                                                              uint32_t *__restrict__ tile_counts, int64_t tiles) {
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x, lane = tid & 63;
+    h[tid] = 0;
+    const int64_t base = (int64_t)blockIdx.x * TILE;
+    W v[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int64_t i = base + r * THREADS + tid;
+        v[r] = w[i < n ? i : n - 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const bool valid = base + r * THREADS + tid < n;
+        const uint64_t live = __ballot(valid);
+        const uint32_t d = (uint32_t)(v[r] >> shift) & 0xff;
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        // One add for the wave when every live lane has the same value
+        // (live lanes are a prefix of the wave).
+        if (__ballot(valid && d == d0) == live) {
+            if (lane == 0 && live) atomicAdd(&h[d0], (uint32_t)__popcll(live));
+        } else if (valid) {
+            atomicAdd(&h[d], 1u);
+        }
+    }
+    __syncthreads();
+    tile_counts[(int64_t)tid * tiles + blockIdx.x] = h[tid];
+}
+
+// Output start of tile t's run of value d: the words of values below d,
+// plus the words of value d in tiles t' < t (tile_counts[d * sd + t' * st]).
+// The words of each value are
+// counts[value], or, with counts null, those of bucket `value` of the rows
+// 0 .. n-1 in buckets of `bucket` rows.  Workgroup d, thread j takes tiles
+// [j*per, (j+1)*per).
+__global__ __launch_bounds__(THREADS) void tile_offsets_kernel(const uint32_t *__restrict__ tile_counts,
+                                                               const uint32_t *__restrict__ counts, int64_t tiles,
+                                                               uint32_t *__restrict__ offsets, int64_t n,
+                                                               int64_t bucket, int64_t sd, int64_t st) {
+    __shared__ uint32_t wt[2][WAVES];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, d = blockIdx.x;
+    // words of values below d
+    uint32_t of_tid;
+    if (counts) {
+        of_tid = counts[tid];
+    } else {
+        const int64_t left = n - tid * bucket;
+        of_tid = (uint32_t)(left <= 0 ? 0 : (left < bucket ? left : bucket));
+    }
+    const uint32_t below_d = wave_sum(tid < d ? of_tid : 0u);
+    const int64_t per = (tiles + THREADS - 1) / THREADS;
+    const uint32_t *row = tile_counts + (int64_t)d * sd;
+    const int64_t t0 = tid * per, t1 = t0 + per < tiles ? t0 + per : tiles;
+    uint32_t mine = 0;
+    for (int64_t t = t0; t < t1; t++) mine += row[t * st];
+    const uint32_t incl = wave_inclusive_sum(mine, lane);
+    if (lane == 63) {
+        wt[0][wv] = incl;
+        wt[1][wv] = below_d;
+    }
+    __syncthreads();
+    uint32_t run = incl - mine, base = 0;
+    for (int w = 0; w < WAVES; w++) {
+        if (w < wv) run += wt[0][w];
+        base += wt[1][w];
+    }
+    run += base;
+    uint32_t *out = offsets + (int64_t)d * tiles;
+    for (int64_t t = t0; t < t1; t++) {
+        out[t] = run;
+        run += row[t * st];
+    }
 }
 
 struct SweepArgs {
@@ -230,31 +220,24 @@ struct SweepArgs {
     uint32_t *r_out;
     int64_t n;
     int shift;
-    unsigned pass;
-    const uint32_t *counts;     // the 256 counts of this byte's values
-    unsigned long long *lb;     // 256 look-back words per tile
-    unsigned *ctr;              // this pass's tile counter
-    unsigned *err;
+    const uint32_t *offsets;    // [256][tiles] output start of each tile's run of each value
+    int64_t tiles;
 };
 
-// One stable pass over the byte at `shift`.
+// One stable pass over the byte at `shift`, tile blockIdx.x.
 __global__ __launch_bounds__(THREADS) void sweep_kernel(SweepArgs a) {
-    // 52224 bytes of LDS: three workgroups per CU.  The tile number and
-    // the scan's wave totals live in `dst` until `dst` is written (after
-    // two more barriers).
+    // 52224 bytes of LDS: three workgroups per CU.  The scan's wave totals
+    // live in `dst` until `dst` is written (after another barrier).
     __shared__ uint64_t sk[TILE];
     __shared__ uint32_t sr[TILE];
     __shared__ uint16_t cnt[WAVES][256];   // per wave: its pairs of each value, then their place (< TILE)
     __shared__ uint32_t dst[256];          // output position of the tile's value-d run, less its LDS start
-    uint32_t *const wt0 = dst, *const wt1 = dst + WAVES;
-    unsigned *const tile_s = dst + 2 * WAVES;
+    uint32_t *const wt = dst;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid == 0) *tile_s = atomicAdd(a.ctr, 1u);
-    const uint32_t count_d = a.counts[tid];   // words of value tid in the whole pass
+    const int64_t tile = blockIdx.x;
+    const uint32_t out_d = a.offsets[(int64_t)tid * a.tiles + tile];   // where this tile's run of value tid starts
 #pragma unroll
     for (int k = 0; k < 4; k++) cnt[wv][k * 64 + lane] = 0;
-    __syncthreads();
-    const int64_t tile = *tile_s;
     const int64_t base = tile * TILE + wv * WAVE_PAIRS;
 
     uint64_t key[KPT];
@@ -299,9 +282,7 @@ __global__ __launch_bounds__(THREADS) void sweep_kernel(SweepArgs a) {
     }
     __syncthreads();
 
-    // 2. Thread d: the tile's count of value d, published at once; the
-    // tile's LDS start of each wave's run of d; the pass's first output
-    // position of d.
+    // 2. Thread d: the tile's LDS start of each wave's run of value d.
     const int d = tid;
     uint32_t c[WAVES], total = 0;
 #pragma unroll
@@ -309,20 +290,11 @@ __global__ __launch_bounds__(THREADS) void sweep_kernel(SweepArgs a) {
         c[w] = cnt[w][d];
         total += c[w];
     }
-    unsigned long long *mine = a.lb + tile * 256 + d;
-    lb_store(mine, lb_word(tile == 0 ? 2 * a.pass + 2 : 2 * a.pass + 1, total));
     const uint32_t incl = wave_inclusive_sum(total, lane);
-    const uint32_t incl_d = wave_inclusive_sum(count_d, lane);
-    if (lane == 63) {
-        wt0[wv] = incl;
-        wt1[wv] = incl_d;
-    }
+    if (lane == 63) wt[wv] = incl;
     __syncthreads();
-    uint32_t start = incl - total, base_d = incl_d - count_d;
-    for (int w = 0; w < wv; w++) {
-        start += wt0[w];
-        base_d += wt1[w];
-    }
+    uint32_t start = incl - total;
+    for (int w = 0; w < wv; w++) start += wt[w];
     const uint32_t tile_start = start;
 #pragma unroll
     for (int w = 0; w < WAVES; w++) {
@@ -331,7 +303,7 @@ __global__ __launch_bounds__(THREADS) void sweep_kernel(SweepArgs a) {
     }
     __syncthreads();
 
-    // 3. The pairs to LDS in sorted order, then the walk back for value d.
+    // 3. The pairs to LDS in sorted order.
 #pragma unroll
     for (int r = 0; r < KPT; r++) {
         const bool valid = base + r * 64 + lane < a.n;
@@ -341,9 +313,7 @@ __global__ __launch_bounds__(THREADS) void sweep_kernel(SweepArgs a) {
             sr[lp] = row[r];
         }
     }
-    const uint32_t before = look_back(a.lb, tile, d, a.pass, a.err);
-    if (tile > 0) lb_store(mine, lb_word(2 * a.pass + 2, before + total));
-    dst[d] = base_d + before - tile_start;
+    dst[d] = out_d - tile_start;
     __syncthreads();
 
     // 4. Runs out: LDS slot i goes to dst[value] + i.
@@ -362,30 +332,77 @@ __global__ __launch_bounds__(THREADS) void sweep_kernel(SweepArgs a) {
     }
 }
 
+// Per tile of sorted positions: how many start a rank (a position whose
+// word differs from the one before), to tile_counts[256][t]; with BUCKETS
+// also its rows in each bucket of 2^bshift rows, to tile_counts[b][t].
+template <bool BUCKETS>
+__global__ __launch_bounds__(THREADS) void starts_kernel(const uint64_t *__restrict__ w,
+                                                         const uint32_t *__restrict__ rows, int64_t n, int bshift,
+                                                         uint32_t *__restrict__ tile_counts, int64_t tiles) {
+    __shared__ uint32_t h[BUCKETS ? 256 : 1];
+    __shared__ uint32_t heads_s;
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (BUCKETS) h[tid] = 0;
+    if (tid == 0) heads_s = 0;
+    const int64_t t0 = (int64_t)blockIdx.x * TILE + tid, last = n - 1;
+    uint64_t w0[KPT], w1[KPT];
+    uint32_t rw[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int64_t p = t0 + r * THREADS;
+        const int64_t pc = p < last ? p : last;
+        w0[r] = w[pc];
+        w1[r] = w[pc > 0 ? pc - 1 : 0];
+        if (BUCKETS) rw[r] = rows[pc];
+    }
+    __syncthreads();
+    uint32_t starts = 0;
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int64_t p = t0 + r * THREADS;
+        starts += p <= last && (p == 0 || w0[r] != w1[r]);
+        if (BUCKETS && p <= last) atomicAdd(&h[rw[r] >> bshift], 1u);
+    }
+    starts = wave_sum(starts);
+    if (lane == 0) atomicAdd(&heads_s, starts);
+    __syncthreads();
+    if (BUCKETS) tile_counts[(int64_t)tid * tiles + blockIdx.x] = h[tid];
+    if (tid == 0) tile_counts[(int64_t)256 * tiles + blockIdx.x] = heads_s;
+}
+
 struct AssignArgs {
     const uint64_t *w;          // sorted words
     const uint32_t *rows;       // their rows
     int64_t n;
     int64_t tiles;
     uint32_t *rank, *first;
-    unsigned long long *lb;
-    unsigned *ctr, *err;
+    const uint32_t *offsets;    // [257][tiles]: rows 0-255 bucket runs, row 256 ranks started before each tile
     uint32_t *n_ranks;
+    int bshift;
+    uint64_t *pairs;            // with BUCKETS: (row << 32 | rank) by buckets of 2^bshift rows
 };
 
 // Ranks from the sorted pairs: a position starts a rank where its word
 // differs from the one before; the rank of a position is the number of
 // starts up to it, less one.  A tile's positions are taken 256 at a time
 // (round r, wave, lane), so loads are coalesced; the starts are counted
-// per wave and round with ballots.
+// per wave and round with ballots.  Without BUCKETS each rank goes
+// straight to its row.  With BUCKETS the (row, rank) pairs are staged in
+// LDS by bucket of rows (in any order inside a bucket) and written as runs,
+// for rows_from_buckets_kernel: 4-byte stores to random rows cost HBM a
+// 32-byte sector each.
+template <bool BUCKETS>
 __global__ __launch_bounds__(THREADS) void assign_kernel(AssignArgs a) {
     __shared__ uint32_t part[KPT * WAVES];   // starts per (round, wave), then their exclusive prefix
-    __shared__ unsigned tile_s;
-    __shared__ uint32_t before_s;
+    __shared__ uint64_t sp[BUCKETS ? TILE : 1];   // staged pairs
+    __shared__ uint32_t bc[BUCKETS ? 256 : 1];    // pairs of each bucket, then the bucket's LDS start
+    __shared__ uint32_t bdst[BUCKETS ? 256 : 1];  // output position of a bucket's run, less its LDS start
+    __shared__ uint32_t bwt[WAVES];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid == 0) tile_s = atomicAdd(a.ctr, 1u);
-    __syncthreads();
-    const int64_t tile = tile_s;
+    const int64_t tile = blockIdx.x;
+    if (BUCKETS) bc[tid] = 0;
+    const uint32_t before = a.offsets[(int64_t)256 * a.tiles + tile];   // ranks started before this tile
+    const uint32_t out_b = BUCKETS ? a.offsets[(int64_t)tid * a.tiles + tile] : 0;
     const int64_t t0 = tile * TILE + tid;
     const int64_t last = a.n - 1;
     const uint64_t below = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
@@ -400,6 +417,13 @@ __global__ __launch_bounds__(THREADS) void assign_kernel(AssignArgs a) {
         row[r] = a.rows[pc];
         heads[r] = __ballot(p <= last && (p == 0 || w != prev));
     }
+    __syncthreads();
+    uint32_t loc[BUCKETS ? KPT : 1];
+    if (BUCKETS) {
+#pragma unroll
+        for (int r = 0; r < KPT; r++)
+            loc[r] = t0 + r * THREADS <= last ? atomicAdd(&bc[row[r] >> a.bshift], 1u) : 0u;
+    }
     if (lane < KPT) {
         uint64_t h = heads[0];
 #pragma unroll
@@ -407,22 +431,26 @@ __global__ __launch_bounds__(THREADS) void assign_kernel(AssignArgs a) {
         part[lane * WAVES + wv] = (uint32_t)__popcll(h);
     }
     __syncthreads();
+    uint32_t bcount = 0, bincl = 0;
+    if (BUCKETS) {
+        bcount = bc[tid];
+        bincl = wave_inclusive_sum(bcount, lane);
+        if (lane == 63) bwt[wv] = bincl;
+    }
     if (wv == 0) {
         const uint32_t v = part[lane];
         const uint32_t incl = wave_inclusive_sum(v, lane);
         part[lane] = incl - v;
-        const uint32_t total = __shfl(incl, 63);
-        unsigned long long *mine = a.lb + tile * 256;
-        if (lane == 0) lb_store(mine, lb_word(tile == 0 ? 2 * ASSIGN_PASS + 2 : 2 * ASSIGN_PASS + 1, total));
-        const uint32_t before = look_back_wave(a.lb, tile, ASSIGN_PASS, a.err, lane);
-        if (lane == 0) {
-            if (tile > 0) lb_store(mine, lb_word(2 * ASSIGN_PASS + 2, before + total));
-            before_s = before;
-            if (tile == a.tiles - 1) *a.n_ranks = before + total;
-        }
+        if (lane == 63 && tile == a.tiles - 1) *a.n_ranks = before + incl;
     }
     __syncthreads();
-    const uint32_t before = before_s;
+    if (BUCKETS) {
+        uint32_t lstart = bincl - bcount;
+        for (int w = 0; w < wv; w++) lstart += bwt[w];
+        bdst[tid] = out_b - lstart;
+        bc[tid] = lstart;
+        __syncthreads();
+    }
 #pragma unroll
     for (int r = 0; r < KPT; r++) {
         const int64_t p = t0 + r * THREADS;
@@ -430,10 +458,56 @@ __global__ __launch_bounds__(THREADS) void assign_kernel(AssignArgs a) {
         // starts up to and including this position, less one
         const uint32_t rk = before + part[r * WAVES + wv] + (uint32_t)__popcll(heads[r] & below) + h - 1;
         if (p <= last) {
-            a.rank[row[r]] = rk;
+            if (BUCKETS)
+                sp[bc[row[r] >> a.bshift] + loc[r]] = ((uint64_t)row[r] << 32) | rk;
+            else
+                a.rank[row[r]] = rk;
             if (h) a.first[rk] = row[r];
         }
     }
+    if (BUCKETS) {
+        __syncthreads();
+        const int len = (int)(a.n - tile * TILE < TILE ? a.n - tile * TILE : TILE);
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            const int i = r * THREADS + tid;
+            const uint64_t v = sp[i];
+            // (slots past the tile's end hold no pair: the mask keeps their
+            // lookup inside bdst)
+            const uint32_t pos = bdst[((uint32_t)(v >> 32) >> a.bshift) & 0xff] + (uint32_t)i;
+            if (i < len) a.pairs[pos] = v;
+        }
+    }
+}
+
+// rank[row] for the rows of bucket blockIdx.x (2^bshift rows, at most
+// 2^15): its pairs, in any order, fill an LDS window of the bucket's ranks,
+// which then goes out in whole lines.
+constexpr int WINDOW = 1 << 15;
+
+__global__ __launch_bounds__(512) void rows_from_buckets_kernel(const uint64_t *__restrict__ pairs, int64_t n,
+                                                                int bshift, uint32_t *__restrict__ rank) {
+    __shared__ uint32_t win[WINDOW];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x << bshift;
+    const int64_t left = n - r0;
+    const int cnt = (int)(left < ((int64_t)1 << bshift) ? left : ((int64_t)1 << bshift));
+    const uint64_t *src = pairs + r0;   // the bucket's pairs: as many as its rows, after those of lower rows
+    for (int i0 = 0; i0 < cnt; i0 += 8 * 512) {
+        uint64_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int i = i0 + k * 512 + tid;
+            v[k] = src[i < cnt ? i : cnt - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int i = i0 + k * 512 + tid;
+            if (i < cnt) win[(uint32_t)((v[k] >> 32) - (uint64_t)r0) & (WINDOW - 1)] = (uint32_t)v[k];
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < cnt; i += 512) rank[r0 + i] = win[i];
 }
 
 // Exclusive scan of 1024 values per block (4 per thread); each block's total
@@ -486,7 +560,6 @@ size_t scan_temp_bytes(size_t n) {
 
 int64_t tiles_of(size_t n) { return (int64_t)((n + TILE - 1) / TILE); }
 
-int hist_blocks_of(int64_t tiles) { return (int)(tiles < HIST_BLOCKS ? (tiles > 0 ? tiles : 1) : HIST_BLOCKS); }
 
 }  // namespace
 
@@ -510,9 +583,8 @@ size_t scan_scratch_bytes(size_t n) { return scan_temp_bytes(n); }
 
 size_t rank_scratch_bytes(size_t n) {
     const int64_t tiles = tiles_of(n);
-    const size_t hb = (size_t)hist_blocks_of(tiles);
-    return round_up(n * 8) * 2 + round_up(n * 4) * 2 + round_up(hb * 2048 * 4) + 2048 * 4 +
-           round_up((size_t)tiles * 256 * 8) + SMALL_WORDS * 4;
+    return round_up(n * 8) * 2 + round_up(n * 4) * 2 + round_up((size_t)tiles * 2048 * 4) + 2048 * 4 +
+           2 * round_up((size_t)tiles * 257 * 4) + SMALL_WORDS * 4;
 }
 
 namespace {
@@ -520,26 +592,23 @@ namespace {
 // Scratch layout shared by ranking and sorting.
 struct RadixScratch {
     uint64_t *wa, *wb;
-    uint32_t *ra, *rb, *partial, *counts;
-    unsigned long long *lb;
+    uint32_t *ra, *rb, *partial, *counts, *tile_counts, *offsets;
     unsigned *small;
-    int64_t tiles, lb_n;
-    int hist_blocks;
+    int64_t tiles;
 };
 
 RadixScratch carve(void *scratch, size_t n) {
     RadixScratch r;
     r.tiles = tiles_of(n);
-    r.lb_n = r.tiles * 256;
-    r.hist_blocks = hist_blocks_of(r.tiles);
     char *p = static_cast<char *>(scratch);
     r.wa = reinterpret_cast<uint64_t *>(p); p += round_up(n * 8);
     r.wb = reinterpret_cast<uint64_t *>(p); p += round_up(n * 8);
     r.ra = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
     r.rb = reinterpret_cast<uint32_t *>(p); p += round_up(n * 4);
-    r.partial = reinterpret_cast<uint32_t *>(p); p += round_up((size_t)r.hist_blocks * 2048 * 4);
+    r.partial = reinterpret_cast<uint32_t *>(p); p += round_up((size_t)r.tiles * 2048 * 4);
     r.counts = reinterpret_cast<uint32_t *>(p); p += 2048 * 4;
-    r.lb = reinterpret_cast<unsigned long long *>(p); p += round_up((size_t)r.lb_n * 8);
+    r.tile_counts = reinterpret_cast<uint32_t *>(p); p += round_up((size_t)r.tiles * 257 * 4);
+    r.offsets = reinterpret_cast<uint32_t *>(p); p += round_up((size_t)r.tiles * 257 * 4);
     r.small = reinterpret_cast<unsigned *>(p);
     return r;
 }
@@ -553,9 +622,9 @@ RadixScratch carve(void *scratch, size_t n) {
 ef_status radix_passes(const uint64_t *words, const uint32_t *rows_in, size_t n, const RadixScratch &x,
                        uint64_t *w_final, uint32_t *r_final, const uint64_t *&w_cur,
                        const uint32_t *&r_cur, int &passes, hipStream_t s) {
-    const HistArgs h{words, (int64_t)n, x.tiles, x.partial, x.lb, x.lb_n, x.small, x.counts};
-    hipLaunchKernelGGL(byte_histograms_kernel, dim3((unsigned)x.hist_blocks), dim3(THREADS), 0, s, h);
-    hipLaunchKernelGGL(sum_histograms_kernel, dim3(HIST_CHUNKS, 8), dim3(THREADS), 0, s, x.partial, x.hist_blocks,
+    const HistArgs h{words, (int64_t)n, x.tiles, x.partial, x.small, x.counts};
+    hipLaunchKernelGGL(byte_histograms_kernel, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, h);
+    hipLaunchKernelGGL(sum_histograms_kernel, dim3(HIST_CHUNKS, 8), dim3(THREADS), 0, s, x.partial, (int)x.tiles,
                        x.counts);
     EF_HIP(hipGetLastError());
     uint32_t counts[2048];
@@ -583,8 +652,18 @@ ef_status radix_passes(const uint64_t *words, const uint32_t *rows_in, size_t n,
             r_next = r_final;
             if (w_final) w_next = w_final;
         }
-        const SweepArgs sa{w_cur, r_cur, w_next, r_next, (int64_t)n, 8 * b, (unsigned)b,
-                           x.counts + 256 * b, x.lb, x.small + b, x.small + S_ERR};
+        if (passes == 0) {
+            // The words in input order: the histogram read counted its tiles.
+            hipLaunchKernelGGL(tile_offsets_kernel, dim3(256), dim3(THREADS), 0, s, x.partial + 256 * b,
+                               x.counts + 256 * b, x.tiles, x.offsets, (int64_t)n, (int64_t)0, (int64_t)1,
+                               (int64_t)2048);
+        } else {
+            hipLaunchKernelGGL(tile_counts_kernel<uint64_t>, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, w_cur,
+                               (int64_t)n, 8 * b, x.tile_counts, x.tiles);
+            hipLaunchKernelGGL(tile_offsets_kernel, dim3(256), dim3(THREADS), 0, s, x.tile_counts,
+                               x.counts + 256 * b, x.tiles, x.offsets, (int64_t)n, (int64_t)0, x.tiles, (int64_t)1);
+        }
+        const SweepArgs sa{w_cur, r_cur, w_next, r_next, (int64_t)n, 8 * b, x.offsets, x.tiles};
         hipLaunchKernelGGL(sweep_kernel, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, sa);
         w_cur = w_next;
         r_cur = r_next;
@@ -596,11 +675,6 @@ ef_status radix_passes(const uint64_t *words, const uint32_t *rows_in, size_t n,
     return EF_OK;
 }
 
-ef_status wait_error(unsigned err) {
-    if (!err) return EF_OK;
-    set_error("radix ranking: a tile waited past its limit for the tiles before it");
-    return EF_DEVICE_ERROR;
-}
 
 }  // namespace
 
@@ -625,12 +699,7 @@ ef_status sort_u64(const uint64_t *words, const uint32_t *rows_in, size_t n, uin
         hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, r_out, (int64_t)n);
     }
     EF_HIP(hipGetLastError());
-    unsigned err = 0;
-    if (passes > 0) {
-        EF_HIP(hipMemcpyAsync(&err, x.small + S_ERR, 4, hipMemcpyDeviceToHost, s));
-        EF_HIP(hipStreamSynchronize(s));
-    }
-    return wait_error(err);
+    return EF_OK;
 }
 
 ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *first,
@@ -660,15 +729,38 @@ ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *fi
         if (n_ranks) *n_ranks = 1;
         return EF_OK;
     }
-    const AssignArgs aa{w_cur, r_cur, (int64_t)n, x.tiles, rank, first, x.lb,
-                        x.small + ASSIGN_PASS, x.small + S_ERR, x.small + S_RANKS};
-    hipLaunchKernelGGL(assign_kernel, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, aa);
+    // Rows in buckets of 2^bshift <= WINDOW rows, at most 256 buckets.
+    int bshift = 0;
+    while (((size_t)256 << bshift) < n) bshift++;
+    const bool buckets = n >= ((size_t)1 << 16) && ((size_t)1 << bshift) <= (size_t)WINDOW;
+    const int64_t nb = buckets ? (int64_t)((n + ((size_t)1 << bshift) - 1) >> bshift) : 0;
+    if (buckets)
+        hipLaunchKernelGGL(starts_kernel<true>, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, w_cur, r_cur, (int64_t)n,
+                           bshift, x.tile_counts, x.tiles);
+    else
+        hipLaunchKernelGGL(starts_kernel<false>, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, w_cur, r_cur,
+                           (int64_t)n, bshift, x.tile_counts, x.tiles);
+    // Row 256 (rank starts): its "values below" are none, so workgroup 0 of
+    // a launch based at row 256 gives the exclusive prefix over tiles.
+    hipLaunchKernelGGL(tile_offsets_kernel, dim3(1), dim3(THREADS), 0, s, x.tile_counts + 256 * x.tiles, x.counts,
+                       x.tiles, x.offsets + 256 * x.tiles, (int64_t)n, (int64_t)0, x.tiles, (int64_t)1);
+    AssignArgs aa{w_cur, r_cur, (int64_t)n, x.tiles, rank, first, x.offsets, x.small + S_RANKS, bshift, nullptr};
+    if (buckets) {
+        hipLaunchKernelGGL(tile_offsets_kernel, dim3((unsigned)nb), dim3(THREADS), 0, s, x.tile_counts,
+                           (const uint32_t *)nullptr, x.tiles, x.offsets, (int64_t)n, (int64_t)1 << bshift, x.tiles,
+                           (int64_t)1);
+        aa.pairs = w_cur == x.wa ? x.wb : x.wa;   // the words buffer the last pass did not write
+        hipLaunchKernelGGL(assign_kernel<true>, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, aa);
+        hipLaunchKernelGGL(rows_from_buckets_kernel, dim3((unsigned)nb), dim3(512), 0, s, aa.pairs, (int64_t)n, bshift,
+                           rank);
+    } else {
+        hipLaunchKernelGGL(assign_kernel<false>, dim3((unsigned)x.tiles), dim3(THREADS), 0, s, aa);
+    }
     EF_HIP(hipGetLastError());
-    unsigned tail[2] = {0, 0};   // error flag, number of ranks
-    EF_HIP(hipMemcpyAsync(tail, x.small + S_ERR, sizeof tail, hipMemcpyDeviceToHost, s));
+    uint32_t ranks = 0;
+    EF_HIP(hipMemcpyAsync(&ranks, x.small + S_RANKS, sizeof ranks, hipMemcpyDeviceToHost, s));
     EF_HIP(hipStreamSynchronize(s));
-    EF_TRY(wait_error(tail[0]));
-    if (n_ranks) *n_ranks = tail[1];
+    if (n_ranks) *n_ranks = ranks;
     return EF_OK;
 }
 
