@@ -51,9 +51,39 @@ struct Item {
 
 struct Counter { unsigned next, pad[3]; };
 
+// Eight consecutive k of one row of a prepared operand, in the element
+// type of precision P: two planes (hi, lo) for BF16x3.  dst is 16-byte
+// aligned (k is padded to whole 64-element steps).
+template <int P>
+__device__ __forceinline__ void put8(typename Elem<P>::T *dst, int64_t plane, const float (&v)[8]) {
+    if constexpr (P == PREC_BF16X3) {
+        bf16x8 h, l;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            __bf16 hh, ll;
+            split_bf16(v[j], hh, ll);
+            h[j] = hh;
+            l[j] = ll;
+        }
+        *reinterpret_cast<bf16x8 *>(dst) = h;
+        *reinterpret_cast<bf16x8 *>(dst + plane) = l;
+    } else if constexpr (P == PREC_BF16) {
+        bf16x8 h;
+#pragma unroll
+        for (int j = 0; j < 8; j++) h[j] = (__bf16)v[j];
+        *reinterpret_cast<bf16x8 *>(dst) = h;
+    } else {
+        *reinterpret_cast<f32x4 *>(dst) = f32x4{v[0], v[1], v[2], v[3]};
+        *reinterpret_cast<f32x4 *>(dst + 4) = f32x4{v[4], v[5], v[6], v[7]};
+    }
+}
+
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
 // Writes one 64 x 64 block of a prepared operand: destination rows r (of m
-// or n) and columns kk (of k).  A is read as [m][k]; B as [k][n] and
-// transposed through LDS.
+// or n) and columns kk (of k), as 8-element chunks (two per thread).  A is
+// read as [m][k]; B as [k][n] and transposed through LDS.  Cells outside
+// the matrix are zeros.
 template <int P>
 __global__ __launch_bounds__(256) void grouped_prep_kernel(const GemmDesc *descs, const PrepTask *tasks,
                                                            typename Elem<P>::T *ops) {
@@ -63,31 +93,37 @@ __global__ __launch_bounds__(256) void grouped_prep_kernel(const GemmDesc *descs
     const GemmDesc d = descs[task.problem];
     const int64_t r0 = (int64_t)task.bi * 64, k0 = (int64_t)task.bj * 64;
     const int tid = threadIdx.x;
+    float v[2][8];
     if (task.which == 0) {
-        const int64_t plane = d.mp * d.kp;
-        E *dst = ops + d.a_off;
-        for (int i = tid; i < 64 * 64; i += 256) {
-            const int r = i / 64, kk = i % 64;
-            const int64_t row = r0 + r, col = k0 + kk;
-            // Unconditional load from a clamped cell, then 0 outside: the
-            // loads issue together instead of one branch and wait each.
-            const float a = d.a[(row < d.m ? row : d.m - 1) * d.k + (col < d.k ? col : d.k - 1)];
-            const float v = (row < d.m && col < d.k) ? a : 0.f;
-            if (row < d.mp && col < d.kp) {
-                if constexpr (P == PREC_BF16X3) {
-                    __bf16 h, l;
-                    split_bf16(v, h, l);
-                    dst[row * d.kp + col] = h;
-                    dst[plane + row * d.kp + col] = l;
-                } else {
-                    dst[row * d.kp + col] = (E)v;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int q = tid + 256 * i;
+            const int64_t row = r0 + q / 8, col = k0 + (q % 8) * 8;
+            if (row >= d.m) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[i][j] = 0.f;
+            } else if (col + 8 <= d.k) {
+                const float *src = d.a + row * d.k + col;
+                const f32x4u x0 = *reinterpret_cast<const f32x4u *>(src);
+                const f32x4u x1 = *reinterpret_cast<const f32x4u *>(src + 4);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    v[i][j] = x0[j];
+                    v[i][j + 4] = x1[j];
+                }
+            } else {
+                // Unconditional loads from clamped cells, then 0 outside:
+                // the loads issue together instead of one branch and wait
+                // each.
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const float a = d.a[row * d.k + (col + j < d.k ? col + j : d.k - 1)];
+                    v[i][j] = col + j < d.k ? a : 0.f;
                 }
             }
         }
     } else {
-        const int64_t plane = d.np * d.kp;
-        E *dst = ops + d.b_off;
-        // Read B[k0 + kk][r0 + r] (coalesced along n), write [r][kk].
+        // Read B[k0 + kk][r0 + r] (coalesced along n) into t[kk][r].
         for (int i = tid; i < 64 * 64; i += 256) {
             const int kk = i / 64, r = i % 64;
             const int64_t kx = k0 + kk, col = r0 + r;
@@ -95,21 +131,21 @@ __global__ __launch_bounds__(256) void grouped_prep_kernel(const GemmDesc *descs
             t[kk][r] = (kx < d.k && col < d.n) ? b : 0.f;
         }
         __syncthreads();
-        for (int i = tid; i < 64 * 64; i += 256) {
-            const int r = i / 64, kk = i % 64;
-            const int64_t row = r0 + r, col = k0 + kk;
-            if (row < d.np && col < d.kp) {
-                const float v = t[kk][r];
-                if constexpr (P == PREC_BF16X3) {
-                    __bf16 h, l;
-                    split_bf16(v, h, l);
-                    dst[row * d.kp + col] = h;
-                    dst[plane + row * d.kp + col] = l;
-                } else {
-                    dst[row * d.kp + col] = (E)v;
-                }
-            }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int q = tid + 256 * i;
+            const int r = q / 8, kc = (q % 8) * 8;
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[i][j] = t[kc + j][r];
         }
+    }
+    const int64_t rows = task.which == 0 ? d.mp : d.np;
+    E *dst = ops + (task.which == 0 ? d.a_off : d.b_off);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int q = tid + 256 * i;
+        const int64_t row = r0 + q / 8, col = k0 + (q % 8) * 8;
+        put8<P>(dst + row * d.kp + col, rows * d.kp, v[i]);
     }
 }
 
