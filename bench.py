@@ -47,9 +47,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-secondary", action="store_true",
-                    help="skip the dense fold and ranking measurements")
+                    help="skip the secondary lines (dense fold, ranking, batched folds, smoother, regridding)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 256 MiB device copy first (a known-size kernel for counter scaling)")
+    ap.add_argument("--line", choices=["dense_fold", "ranking", "batched_folds", "smoother", "regridding"],
+                    help="print one secondary line alone (for profiling one workload) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -68,6 +70,12 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.line:
+        one = {"dense_fold": lambda: dense_fold_line(dev, not args.no_cpu), "ranking": lambda: ranking_line(dev),
+               "batched_folds": lambda: batched_folds_line(dev), "smoother": smoother_line,
+               "regridding": regrid_line}[args.line]
+        print(json.dumps({args.line: one()}), flush=True)
+        return
     q64, k64, v64 = data.attention_tables(H, T, D, seed=3)
     g64 = data.normal(3, 4, H * T * D).reshape(H, T, D)
     to = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x)).to(dt).to(dev)
@@ -158,7 +166,8 @@ def main():
     secondary = None
     if not args.no_secondary:
         secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev),
-                     "batched_folds": batched_folds_line(dev)}
+                     "batched_folds": batched_folds_line(dev), "smoother": smoother_line(),
+                     "regridding": regrid_line()}
 
     line = {
         "metric": "fold GFLOP/s + attention fwd+bwd ms: mixed vs plain FP32, 1 MI355X",
@@ -277,6 +286,75 @@ def batched_folds_line(dev):
                            "rel_err_fold0": err}
         plan.close()
     return line
+
+
+def smoother_line():
+    """Gaussian kernel smoothing (config 5) as the fused pipeline's general
+    FP64 path: scores -power(x - y, 2)/h over the cross join of x (20k rows)
+    and y (20k rows), values U[0, 100] (seed 5), h = 4, weights exp with the
+    kept maximum shift, normalised per kept group, applied to w(j) ~ N(0, 1).
+    Arrow tables in host memory in, Arrow out (transfers included)."""
+    import time
+    import pyarrow as pa
+    from einfold_amd import data, ir, kernels
+    n = m = 20000
+    xs = pa.table({"i": np.arange(n, dtype=np.int64), "x": data.unit(5, 1, n) * 100.0})
+    ys = pa.table({"j": np.arange(m, dtype=np.int64), "y": data.unit(5, 2, m) * 100.0})
+    ws = pa.table({"j": np.arange(m, dtype=np.int64), "w": data.normal(5, 3, m)})
+    x = ir.Expression.col(0, 1)
+    y = ir.Expression.col(1, 1)
+    score = -((x - y) ** ir.Expression.lit(2.0)) / ir.Expression.lit(4.0)
+    producer = ir.Fold([ir.KeyEquality.EQUAL] * 2, [ir.Operand((ir.Key(0, 0),), None),
+                                                    ir.Operand((ir.Key(1, 0),), None)],
+                       [ir.Output(0), ir.Output(1)], row=score)
+    p = ir.Pipeline(producer, ir.Exp(ir.Scale(), ir.Shift.KEPT_MAXIMUM), [], ir.Operand((ir.Key(1, 0),), 1), [0],
+                    ir.Normalizer.KEPT_GROUPS)
+    out = kernels.pipeline(p, xs, ys, ws)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = kernels.pipeline(p, xs, ys, ws)
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    rows = sum(b.num_rows for b in out.batches)
+    return {"workload": "Gaussian smoother 20k x 20k, h=4 (general FP64 pipeline path)", "pairs": n * m,
+            "rows_out": rows, "ms_arrow_to_arrow": round(ms, 2),
+            "gpairs_per_s": round(n * m / (ms * 1e-3) / 1e9, 2)}
+
+
+def regrid_line():
+    """Regridding (config 5) as a sorting-path fold: out(dst, t) = SUM(w.w *
+    x.v) FROM w JOIN x ON w.src = x.src GROUP BY dst, t, at a tenth of the
+    stated scale: 100k dst cells with 16 random src each over 100k src
+    cells, 365 time steps (1.6M weight rows, 36.5M value rows, 5.84e8 FMA,
+    36.5M output groups).  Int32 keys; the factors are Float64 (the
+    reference takes Float64 factors only).  Arrow in host memory in, Arrow
+    out (transfers included)."""
+    import time
+    import pyarrow as pa
+    from einfold_amd import data, ir, kernels
+    n_dst, per, n_src, steps = 100_000, 16, 100_000, 365
+    dst = np.repeat(np.arange(n_dst, dtype=np.int32), per)
+    src = (data.next_u64(7, 1, n_dst * per) % np.uint64(n_src)).astype(np.int32)
+    w = pa.table({"dst": dst, "src": src, "w": data.unit(7, 2, n_dst * per)})
+    xs = pa.table({"src": np.repeat(np.arange(n_src, dtype=np.int32), steps),
+                   "t": np.tile(np.arange(steps, dtype=np.int32), n_src),
+                   "v": data.normal(7, 3, n_src * steps)})
+    eq = ir.KeyEquality.EQUAL
+    f = ir.Fold([eq] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2), ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)],
+                [ir.Output(0), ir.Output(2)])
+    out = kernels.fold(f, w, xs)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = kernels.fold(f, w, xs)
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    rows = sum(b.num_rows for b in out.batches)
+    fma = n_dst * per * steps
+    return {"workload": "regridding 100k dst x 16 src, 365 steps (a tenth of 1M dst)", "algorithm": out.algorithm,
+            "fma": fma, "rows_out": rows, "ms_arrow_to_arrow": round(ms, 2),
+            "gflop_per_s": round(2 * fma / (ms * 1e-3) / 1e9, 2)}
 
 
 def ranking_line(dev):

@@ -272,6 +272,77 @@ __global__ __launch_bounds__(256) void gustavson_kernel(GustavsonArgs g) {
         __builtin_amdgcn_wave_barrier();
     }
 }
+
+// The same product with few output columns (ny <= YCAP): each wave's
+// accumulator and states live in LDS, rows are ordered by a wave-level
+// sync instead of a memory fence, and an x's groups come out by a scan of
+// the states in y order (the order of the sorted list above).  Waves take
+// x's XCHUNK at a time.  The sums are the same, in the same order.
+constexpr int XCHUNK = 8;
+
+template <int YCAP>
+__global__ __launch_bounds__(256) void gustavson_lds_kernel(GustavsonArgs g) {
+    __shared__ double acc_s[4][YCAP];
+    __shared__ uint8_t state_s[4][YCAP];   // bit 0 reached, bit 1 valued
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double *acc = acc_s[wv];
+    uint8_t *state = state_s[wv];
+    const uint64_t below = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    for (int64_t y = lane; y < g.ny; y += 64) {
+        acc[y] = 0.0;
+        state[y] = 0;
+    }
+    wave_sync_lds();
+    for (;;) {
+        int64_t x0 = 0;
+        if (lane == 0) x0 = (int64_t)atomicAdd(g.next, (unsigned long long)XCHUNK);
+        x0 = __shfl(x0, 0);
+        if (x0 >= g.nx) break;
+        const int64_t x1 = x0 + XCHUNK < g.nx ? x0 + XCHUNK : g.nx;
+        for (int64_t x = x0; x < x1; x++) {
+            for (uint32_t i = g.a_start[x]; i < g.a_start[x + 1]; i++) {
+                const uint32_t z = g.a_z[i];
+                const bool av = g.a_valid[i] != 0;
+                const double p = g.a_value[i];
+                const uint32_t bs = g.b_start[z], be = g.b_start[z + 1];
+                for (uint32_t j = bs + lane; j < be; j += 64) {
+                    const uint32_t y = g.b_y[j];
+                    if (j != bs && g.b_y[j - 1] == y) continue;   // not the head of its run
+                    uint8_t st = state[y] | 1;
+                    double a = acc[y];
+                    for (uint32_t k = j; k < be && g.b_y[k] == y; k++) {
+                        if (av && g.b_valid[k]) {
+                            a = a + p * g.b_value[k];
+                            st |= 2;
+                        }
+                    }
+                    acc[y] = a;
+                    state[y] = st;
+                }
+                // This row's sums land before the next row's lanes read them.
+                wave_sync_lds();
+            }
+            const uint64_t base = g.bound_off[x];
+            uint32_t placed = 0;
+            for (int64_t y0 = 0; y0 < g.ny; y0 += 64) {
+                const int64_t y = y0 + lane;
+                const uint8_t st = y < g.ny ? state[y] : 0;
+                const uint64_t m = __ballot(st & 1);
+                if (st & 1) {
+                    const uint64_t o = base + placed + __popcll(m & below);
+                    g.g_y[o] = (uint32_t)y;
+                    g.g_sum[o] = acc[y];
+                    g.g_valued[o] = (st >> 1) & 1;
+                    acc[y] = 0.0;
+                    state[y] = 0;
+                }
+                placed += (uint32_t)__popcll(m);
+            }
+            if (lane == 0) g.g_count[x] = placed;
+            wave_sync_lds();
+        }
+    }
+}
 #pragma clang fp contract(on)
 
 // ---- output ----------------------------------------------------------------
@@ -528,7 +599,12 @@ ef_status group_bounds(const uint32_t *a_start, const uint32_t *a_z, const uint3
 
 ef_status gustavson(const GustavsonArgs &g, int blocks, hipStream_t s) {
     if (g.nx == 0) return EF_OK;
-    hipLaunchKernelGGL(gustavson_kernel, dim3(blocks), dim3(256), 0, s, g);
+    if (g.ny <= 512) {
+        const int64_t want = (g.nx + 4 * XCHUNK - 1) / (4 * XCHUNK);
+        hipLaunchKernelGGL(gustavson_lds_kernel<512>, dim3((unsigned)(want < 2048 ? want : 2048)), dim3(256), 0, s, g);
+    } else {
+        hipLaunchKernelGGL(gustavson_kernel, dim3(blocks), dim3(256), 0, s, g);
+    }
     EF_HIP(hipGetLastError());
     return EF_OK;
 }
