@@ -99,39 +99,43 @@ def main():
 
     P = lambda t: ctypes.c_void_p(t.data_ptr())
 
-    def step(desc, q, k, v, g, bf, kv_events=None):
+    def step(desc, q, k, v, g, bf):
+        """One training step of the pipeline: the forward (out and the row
+        statistics), then the backward (dq, dk, dv) as one entry point."""
         L = lib()
         if "scratch" not in bf:
             bf["scratch"] = device.attention_scratch(desc, dev)
         sc = P(bf["scratch"])
         check(L.ef_attention_fwd(ctypes.byref(desc), P(q), P(k), P(v), P(bf["out"]), P(bf["m"]),
                                  P(bf["l"]), None, sc, stream))
+        check(L.ef_attention_bwd(ctypes.byref(desc), P(q), P(k), P(v), P(bf["out"]), P(g), P(bf["m"]),
+                                 P(bf["l"]), P(bf["dq"]), P(bf["dk"]), P(bf["dv"]), sc, stream))
+
+    def kv_stage_ms(desc, q, k, v, g, bf, reps):
+        """The dK/dV stage alone (its counter reset, the kernel and the sum of
+        its parts), after a full step filled the scratch: the roofline's
+        kernel, timed by events around ef_attention_bwd_kv."""
+        L = lib()
+        sc = P(bf["scratch"])
         check(L.ef_attention_bwd_prepare(ctypes.byref(desc), P(q), P(k), P(v), P(bf["out"]), P(g), sc, stream))
-        if kv_events:
-            kv_events[0].record()
-        check(L.ef_attention_bwd_kv(ctypes.byref(desc), P(k), P(v), P(bf["m"]), P(bf["l"]), sc,
-                                    P(bf["dk"]), P(bf["dv"]), stream))
-        if kv_events:
-            kv_events[1].record()
-        check(L.ef_attention_bwd_q(ctypes.byref(desc), P(q), P(g), P(bf["m"]), P(bf["l"]), sc,
-                                   P(bf["dq"]), stream))
+        run = lambda: check(L.ef_attention_bwd_kv(ctypes.byref(desc), P(k), P(v), P(bf["m"]), P(bf["l"]), sc,
+                                                  P(bf["dk"]), P(bf["dv"]), stream))
+        return _events(run, reps)
 
     def timed(desc, q, k, v, g, bf, steps, warmup, kv_time=False):
         for _ in range(warmup):
             step(desc, q, k, v, g, bf)
         torch.cuda.synchronize()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(steps)] if kv_time else [None] * steps
         t0 = time.perf_counter()
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         start.record()
         for i in range(steps):
-            step(desc, q, k, v, g, bf, evs[i])
+            step(desc, q, k, v, g, bf)
         stop.record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ms = start.elapsed_time(stop) / steps
-        kv_ms = float(np.mean([a.elapsed_time(b) for a, b in evs])) if kv_time else None
+        kv_ms = kv_stage_ms(desc, q, k, v, g, bf, steps) if kv_time else None
         return ms, kv_ms, wall
 
     # FP64 reference (the GPU's FP64 form), once.
