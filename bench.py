@@ -47,10 +47,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-secondary", action="store_true",
-                    help="skip the secondary lines (dense fold, ranking, batched folds, smoother, regridding)")
+                    help="skip the secondary lines (dense fold, ranking, batched and large folds, smoother, "
+                         "regridding)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 256 MiB device copy first (a known-size kernel for counter scaling)")
-    ap.add_argument("--line", choices=["dense_fold", "ranking", "batched_folds", "smoother", "regridding"],
+    ap.add_argument("--line", choices=["dense_fold", "ranking", "batched_folds", "large_fold", "smoother",
+                                       "regridding"],
                     help="print one secondary line alone (for profiling one workload) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -72,7 +74,8 @@ def main():
         del src, dst
     if args.line:
         one = {"dense_fold": lambda: dense_fold_line(dev, not args.no_cpu), "ranking": lambda: ranking_line(dev),
-               "batched_folds": lambda: batched_folds_line(dev), "smoother": smoother_line,
+               "batched_folds": lambda: batched_folds_line(dev), "large_fold": lambda: large_fold_line(dev),
+               "smoother": smoother_line,
                "regridding": regrid_line}[args.line]
         print(json.dumps({args.line: one()}), flush=True)
         return
@@ -170,7 +173,8 @@ def main():
     secondary = None
     if not args.no_secondary:
         secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev),
-                     "batched_folds": batched_folds_line(dev), "smoother": smoother_line(),
+                     "batched_folds": batched_folds_line(dev), "large_fold": large_fold_line(dev),
+                     "smoother": smoother_line(),
                      "regridding": regrid_line()}
 
     line = {
@@ -288,6 +292,34 @@ def batched_folds_line(dev):
                            "frac_of_peak": round(tf / (peak / 1e12), 4),
                            "issued_frac": round((3 if precision == "bf16x3" else 1) * tf / (peak / 1e12), 4),
                            "rel_err_fold0": err}
+        plan.close()
+    return line
+
+
+def large_fold_line(dev):
+    """One large dense Float32 fold, 8192 x 8192 x 8192 (2 M N K = 1.1e12
+    FLOP), through the grouped product with 128 x 128 tiles, in each
+    precision: FP32 (FP32 MFMA), BF16x3 (three BF16 products) and BF16.
+    Fractions are of the MFMA peak of the arithmetic issued."""
+    import torch
+    from einfold_amd import device
+    n = 8192
+    g = torch.Generator().manual_seed(4)
+    a = torch.randn(n, n, generator=g).to(dev)
+    b = torch.randn(n, n, generator=g).to(dev)
+    c = torch.empty(n, n, device=dev)
+    flop = 2.0 * n * n * n
+    line = {"workload": "dense FP32 fold 8192 x 8192 x 8192", "tflop": round(flop / 1e12, 3)}
+    rows = torch.arange(0, n, 1024, device=dev)
+    want = a[rows].double() @ b.double()
+    for precision, issued, peak in (("fp32", 1, PEAK_FP32), ("bf16x3", 3, PEAK_BF16), ("bf16", 1, PEAK_BF16)):
+        plan = device.GroupedPlan([(a, b, c)], precision, 128)
+        ms = _events(plan.run, 5)
+        err = float(torch.linalg.vector_norm(c[rows].double() - want) / torch.linalg.vector_norm(want))
+        tf = flop / (ms * 1e-3) / 1e12
+        line[precision] = {"ms": round(ms, 3), "tflops": round(tf, 1),
+                           "frac_of_issued_peak": round(issued * tf / (peak / 1e12), 4),
+                           "rel_err_8_rows": err}
         plan.close()
     return line
 

@@ -29,7 +29,6 @@ namespace ef {
 namespace {
 
 constexpr int KS = 64;          // k per step
-constexpr int PADT = 128;       // rows and columns padded to this
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 struct GemmDesc {
@@ -49,7 +48,6 @@ struct Item {
     int problem, tm, tn, pad;
 };
 
-struct Counter { unsigned next, pad[3]; };
 
 // Eight consecutive k of one row of a prepared operand, in the element
 // type of precision P: two planes (hi, lo) for BF16x3.  dst is 16-byte
@@ -149,41 +147,60 @@ __global__ __launch_bounds__(256) void grouped_prep_kernel(const GemmDesc *descs
     }
 }
 
-__global__ void grouped_zero_kernel(Counter *c) {
-    if (threadIdx.x == 0) c->next = 0;
-}
 
-// One workgroup of 4 waves computes a BM x BN output tile; wave (wr, wc)
-// owns a (BM/2) x (BN/2) quarter as 16 x 16 MFMA blocks.
-template <int P, int BM, int BN>
-__global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, int n_items,
-                                                              const typename Elem<P>::T *ops, Counter *ctr) {
+// Workgroups of the GEMM that fit a CU's 160 KB of LDS (at most 4), and
+// the waves per SIMD that makes: the registers are held to match, so that
+// one workgroup's barrier and LDS staging overlap another's MFMAs.
+// LDS row pitch in elements: 16-byte fragment reads of BF16 rows conflict
+// at a 144-byte pitch and not at 160 bytes (measured); other rows carry
+// one 16-byte chunk of padding.
+template <int P, int KSTEP> struct GemmPitch {
+    static constexpr int value =
+        sizeof(typename Elem<P>::T) == 2 && KSTEP == 64 ? KSTEP + 16 : KSTEP + 16 / (int)sizeof(typename Elem<P>::T);
+};
+
+template <int P, int BM, int BN, int WAVES, int KSTEP> struct GemmOcc {
+    static constexpr int PITCH = GemmPitch<P, KSTEP>::value;
+    static constexpr int LDS = 2 * Elem<P>::planes * (BM + BN) * PITCH * (int)sizeof(typename Elem<P>::T);
+    static constexpr int groups = 163840 / LDS < 4 ? 163840 / LDS : 4;
+    static constexpr int per_simd = groups * WAVES / 4 > 0 ? groups * WAVES / 4 : 1;
+};
+
+// One workgroup of WGM x WGN waves computes a BM x BN output tile; wave
+// (wr, wc) owns a (BM/WGM) x (BN/WGN) block as 16 x 16 MFMA blocks, k in
+// steps of KSTEP.  Larger tiles stage fewer operand bytes per product:
+// 256 x 256 does 128 FLOP per staged byte, 128 x 128 64.
+template <int P, int BM, int BN, int WGM, int WGN, int KSTEP>
+__global__ __launch_bounds__(64 * WGM * WGN)
+__attribute__((amdgpu_waves_per_eu(GemmOcc<P, BM, BN, WGM * WGN, KSTEP>::per_simd)))
+void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, int n_items, const typename Elem<P>::T *ops) {
     typedef typename Elem<P>::T E;
     typedef typename Acc<P>::type AccV;
+    constexpr int NT = 64 * WGM * WGN;
     constexpr int PL = Elem<P>::planes;
     constexpr int CE = 16 / sizeof(E);
-    constexpr int PITCH = KS + CE;   // one chunk of padding per row
+    constexpr int PITCH = GemmPitch<P, KSTEP>::value;
     constexpr int APLANE = BM * PITCH, BPLANE = BN * PITCH;
     constexpr int BUF = PL * (APLANE + BPLANE);
-    constexpr int ACH = PL * BM * (KS / CE), BCH = PL * BN * (KS / CE);
-    constexpr int AN = ACH / 256, BNN = BCH / 256, N = AN + BNN;
-    constexpr int WM = BM / 2, WN = BN / 2, IM = WM / 16, JN = WN / 16;
-    constexpr int STEPS = KS / 32;   // 32-wide fragment chunks per k step
+    constexpr int ACH = PL * BM * (KSTEP / CE), BCH = PL * BN * (KSTEP / CE);
+    constexpr int AN = ACH / NT, BNN = BCH / NT, N = AN + BNN;
+    constexpr int WM = BM / WGM, WN = BN / WGN, IM = WM / 16, JN = WN / 16;
+    constexpr int STEPS = KSTEP / 32;   // 32-wide fragment chunks per k step
+    static_assert(ACH % NT == 0 && BCH % NT == 0 && KSTEP % 32 == 0 && 64 % KSTEP == 0, "tile shape");
     __shared__ __attribute__((aligned(16))) E lds[2 * BUF];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
-    const int wr = w >> 1, wc = w & 1;
+    const int wr = w / WGN, wc = w % WGN;
     // This thread's chunks: (plane, row, col) of the A and B tiles.
     int crow_[N], ccol[N], cpl[N];
 #pragma unroll
     for (int i = 0; i < N; i++) {
-        const int kq = tid + 256 * (i < AN ? i : i - AN);
-        const int per_row = KS / CE;
+        const int kq = tid + NT * (i < AN ? i : i - AN);
+        const int per_row = KSTEP / CE;
         const int rows = i < AN ? BM : BN;
         cpl[i] = kq / (rows * per_row);
         crow_[i] = (kq % (rows * per_row)) / per_row;
         ccol[i] = (kq % per_row) * CE;
     }
-    (void)ctr;
     // Items are sorted heaviest first; workgroup g takes g, g + grid, ...
     // (the hardware starts workgroups in index order as CUs free up).
     for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
@@ -209,7 +226,7 @@ __global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *de
         E *dst_ = (BUFP) + (i_ < AN ? cpl[i_] * APLANE : PL * APLANE + cpl[i_] * BPLANE);             \
         *reinterpret_cast<u32x4 *>(dst_ + crow_[i_] * PITCH + ccol[i_]) = pre[i_];                    \
     }
-        const int64_t nk = d.kp / KS;   // 0 for k = 0: no loads, C = 0
+        const int64_t nk = d.kp / KSTEP;   // 0 for k = 0: no loads, C = 0
         if (nk > 0) {
             EF_GLOAD(0);
             EF_LSTORE(lds);
@@ -217,7 +234,7 @@ __global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *de
         __syncthreads();
         for (int64_t ks = 0; ks < nk; ks++) {
             const int cur = (int)(ks & 1);
-            if (ks + 1 < nk) EF_GLOAD((ks + 1) * KS);
+            if (ks + 1 < nk) EF_GLOAD((ks + 1) * KSTEP);
             const E *as = lds + cur * BUF;
             const E *bs = as + PL * APLANE;
 #pragma unroll
@@ -257,7 +274,14 @@ __global__ __launch_bounds__(256, 1) void grouped_gemm_kernel(const GemmDesc *de
 
 size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int tile_rows(int config) { return config == 0 ? 128 : 64; }
+// Tile shapes: config 0 is 128 x 128, 1 is 64 x 64, 2 the large shape of
+// the BF16 precisions (256 x 256 for BF16; 256 x 128 with 32-wide k steps
+// for BF16x3, whose two planes would not fit LDS twice at 256 x 256).
+bool config_valid(uint32_t precision, int config) { return config < 2 || precision != EF_PRECISION_FP32; }
+int tile_m(int config) { return config == 0 ? 128 : config == 1 ? 64 : 256; }
+int tile_n(uint32_t precision, int config) {
+    return config == 0 ? 128 : config == 1 ? 64 : (precision == EF_PRECISION_BF16 ? 256 : 128);
+}
 
 }  // namespace
 
@@ -267,12 +291,11 @@ int tile_rows(int config) { return config == 0 ? 128 : 64; }
 struct GroupedPlan {
     std::vector<ef_gemm_problem> problems;
     uint32_t precision = EF_PRECISION_BF16X3;
-    int config = 0;                   // tile shape: 0 = 128 x 128, 1 = 64 x 64
+    int config = 0;                   // tile shape: 0 = 128 x 128, 1 = 64 x 64, 2 = large (BF16 precisions)
     DeviceBuffer mem;
     GemmDesc *descs = nullptr;
     PrepTask *tasks = nullptr;
     Item *items = nullptr;
-    Counter *ctr = nullptr;
     void *ops = nullptr;
     int n_tasks = 0, n_items = 0, grid = 0;
     hipGraph_t graph = nullptr;
@@ -294,7 +317,7 @@ int cus() {
 }
 
 ef_status build(GroupedPlan &p, hipStream_t s) {
-    const int T = tile_rows(p.config);
+    const int TM = tile_m(p.config), TN = tile_n(p.precision, p.config);
     const size_t esz = p.precision == EF_PRECISION_FP32 ? 4 : 2;
     const size_t planes = p.precision == EF_PRECISION_BF16X3 ? 2 : 1;
     std::vector<GemmDesc> descs;
@@ -306,8 +329,11 @@ ef_status build(GroupedPlan &p, hipStream_t s) {
         GemmDesc d;
         d.a = q.a; d.b = q.b; d.c = q.c;
         d.m = (int64_t)q.m; d.n = (int64_t)q.n; d.k = (int64_t)q.k;
-        d.mp = (d.m + PADT - 1) / PADT * PADT;
-        d.np = (d.n + PADT - 1) / PADT * PADT;
+        // rows padded to whole tiles (of at least 128, the prep's 64 x 64
+        // blocks in whole pairs), k to whole 64-wide steps
+        const int64_t pm = TM < 128 ? 128 : TM, pn = TN < 128 ? 128 : TN;
+        d.mp = (d.m + pm - 1) / pm * pm;
+        d.np = (d.n + pn - 1) / pn * pn;
         d.kp = (d.k + KS - 1) / KS * KS;
         d.a_off = off; off += (int64_t)planes * d.mp * d.kp;
         d.b_off = off; off += (int64_t)planes * d.np * d.kp;
@@ -317,21 +343,20 @@ ef_status build(GroupedPlan &p, hipStream_t s) {
             for (int bj = 0; bj < d.kp / 64; bj++) tasks.push_back({(int)i, 0, bi, bj});
         for (int bi = 0; bi < d.np / 64; bi++)
             for (int bj = 0; bj < d.kp / 64; bj++) tasks.push_back({(int)i, 1, bi, bj});
-        for (int tm = 0; tm < (d.m + T - 1) / T; tm++)
-            for (int tn = 0; tn < (d.n + T - 1) / T; tn++) items.push_back({d.kp, Item{(int)i, tm, tn, 0}});
+        for (int tm = 0; tm < (d.m + TM - 1) / TM; tm++)
+            for (int tn = 0; tn < (d.n + TN - 1) / TN; tn++) items.push_back({d.kp, Item{(int)i, tm, tn, 0}});
     }
     // Largest k first; ties in problem and tile order (deterministic list).
     std::stable_sort(items.begin(), items.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
     p.n_tasks = (int)tasks.size();
     p.n_items = (int)items.size();
     const size_t bytes = up256(descs.size() * sizeof(GemmDesc)) + up256(tasks.size() * sizeof(PrepTask)) +
-                         up256(items.size() * sizeof(Item)) + 256 + up256((size_t)off * esz);
+                         up256(items.size() * sizeof(Item)) + up256((size_t)off * esz);
     EF_TRY(p.mem.alloc(bytes));
     char *at = static_cast<char *>(p.mem.ptr);
     p.descs = reinterpret_cast<GemmDesc *>(at); at += up256(descs.size() * sizeof(GemmDesc));
     p.tasks = reinterpret_cast<PrepTask *>(at); at += up256(tasks.size() * sizeof(PrepTask));
     p.items = reinterpret_cast<Item *>(at); at += up256(items.size() * sizeof(Item));
-    p.ctr = reinterpret_cast<Counter *>(at); at += 256;
     p.ops = at;
     std::vector<Item> flat;
     for (auto &x : items) flat.push_back(x.second);
@@ -350,12 +375,22 @@ ef_status launch_steps(const GroupedPlan &p, hipStream_t s) {
     typedef typename Elem<P>::T E;
     if (p.n_items == 0) return EF_OK;
     hipLaunchKernelGGL(grouped_prep_kernel<P>, dim3((unsigned)p.n_tasks), dim3(256), 0, s, p.descs, p.tasks, (E *)p.ops);
-    if (p.config == 0)
-        hipLaunchKernelGGL((grouped_gemm_kernel<P, 128, 128>), dim3((unsigned)p.grid), dim3(256), 0, s, p.descs,
-                           p.items, p.n_items, (const E *)p.ops, p.ctr);
-    else
-        hipLaunchKernelGGL((grouped_gemm_kernel<P, 64, 64>), dim3((unsigned)p.grid), dim3(256), 0, s, p.descs,
-                           p.items, p.n_items, (const E *)p.ops, p.ctr);
+    if (p.config == 0) {
+        hipLaunchKernelGGL((grouped_gemm_kernel<P, 128, 128, 2, 2, 64>), dim3((unsigned)p.grid), dim3(256), 0, s,
+                           p.descs, p.items, p.n_items, (const E *)p.ops);
+    } else if (p.config == 1) {
+        hipLaunchKernelGGL((grouped_gemm_kernel<P, 64, 64, 2, 2, 64>), dim3((unsigned)p.grid), dim3(256), 0, s,
+                           p.descs, p.items, p.n_items, (const E *)p.ops);
+    } else if constexpr (P == PREC_BF16) {
+        hipLaunchKernelGGL((grouped_gemm_kernel<P, 256, 256, 2, 4, 64>), dim3((unsigned)p.grid), dim3(512), 0, s,
+                           p.descs, p.items, p.n_items, (const E *)p.ops);
+    } else if constexpr (P == PREC_BF16X3) {
+        hipLaunchKernelGGL((grouped_gemm_kernel<P, 256, 128, 4, 2, 32>), dim3((unsigned)p.grid), dim3(512), 0, s,
+                           p.descs, p.items, p.n_items, (const E *)p.ops);
+    } else {
+        set_error("no large tile shape for FP32 products");
+        return EF_INVALID;
+    }
     EF_HIP(hipGetLastError());
     return EF_OK;
 }
@@ -415,8 +450,8 @@ ef_status grouped_plan_create(const ef_gemm_problem *problems, size_t count, uin
     plan->problems.assign(problems, problems + count);
     plan->precision = precision;
     ef_status st = EF_OK;
-    if (tile == 128 || tile == 64) {
-        plan->config = tile == 128 ? 0 : 1;
+    if (tile == 128 || tile == 64 || (tile == 256 && config_valid(precision, 2))) {
+        plan->config = tile == 128 ? 0 : tile == 64 ? 1 : 2;
     } else {
         const auto sig = signature(*plan);
         int known = -1;
@@ -430,7 +465,8 @@ ef_status grouped_plan_create(const ef_gemm_problem *problems, size_t count, uin
         } else {
             float best = 1e30f;
             int best_cfg = 0;
-            for (int cfg = 0; cfg < 2 && st == EF_OK; cfg++) {
+            for (int cfg = 0; cfg < 3 && st == EF_OK; cfg++) {
+                if (!config_valid(precision, cfg)) continue;
                 plan->config = cfg;
                 float ms = 0;
                 st = time_config(*plan, s, ms);
@@ -482,7 +518,7 @@ ef_status grouped_plan_run(GroupedPlan *plan, void *stream) {
     return EF_OK;
 }
 
-int grouped_plan_tile(const GroupedPlan *plan) { return plan ? tile_rows(plan->config) : 0; }
+int grouped_plan_tile(const GroupedPlan *plan) { return plan ? tile_m(plan->config) : 0; }
 
 void grouped_plan_destroy(GroupedPlan *plan) { delete plan; }
 

@@ -160,8 +160,9 @@ def attention_bwd(q, k, v, out, dout, row_max, row_sum, scale: float, causal: bo
 
 class GroupedPlan:
     """Many uneven products c = a @ b (float32 tensors on the GPU, row-major)
-    in one persistent launch replayed from a HIP graph.  ``tile`` 64 or 128
-    fixes the tile shape; 0 lets the autotuner choose.  The plan holds the
+    in one persistent launch replayed from a HIP graph.  ``tile`` 64, 128 or
+    (BF16 and BF16x3 only) 256 fixes the tile shape; 0 lets the autotuner
+    choose.  The plan holds the
     tensors' addresses: keep them alive, and build a new plan if they move."""
 
     def __init__(self, problems, precision: str = "bf16x3", tile: int = 0):

@@ -342,7 +342,9 @@ ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const
  * with Float32 factors, dense.rs:706-770): c = a b per problem, row-major
  * a [m][k], b [k][n], c [m][n], device pointers.  A plan prepares the
  * problems once (work list, operand buffers, the tile shape its autotuner
- * measured fastest for this set of shapes unless `tile` is 64 or 128) and
+ * measured fastest for this set of shapes unless `tile` is 64, 128 or, for
+ * the BF16 precisions, 256: 256 x 256 tiles for BF16, 256 x 128 for
+ * BF16x3, for large products) and
  * captures prep + GEMM as a HIP graph; each run replays it on `stream`.
  * The plan reads the problems' pointers at every run and must be rebuilt
  * when they change.  precision: EF_PRECISION_FP32, _BF16X3 or _BF16. */
@@ -356,7 +358,7 @@ typedef struct ef_grouped_plan ef_grouped_plan;
 ef_status ef_grouped_plan_create(const ef_gemm_problem *problems, size_t count, uint32_t precision, int tile,
                                  void *stream, ef_grouped_plan **plan);
 ef_status ef_grouped_plan_run(ef_grouped_plan *plan, void *stream);
-/* The plan's tile edge (64 or 128). */
+/* The plan's tile edge in rows (64, 128 or 256). */
 int ef_grouped_plan_tile(const ef_grouped_plan *plan);
 void ef_grouped_plan_destroy(ef_grouped_plan *plan);
 

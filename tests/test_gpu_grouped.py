@@ -46,21 +46,50 @@ def test_bits_independent_of_tile_and_replay(cuda):
     shapes = [(100, 200, 300), (33, 17, 1000), (256, 256, 64), (1, 5, 7), (130, 129, 31)]
     p64 = _problems(shapes, cuda, seed=1)
     p128 = [(a, b, torch.empty_like(c)) for a, b, c in p64]
+    p256 = [(a, b, torch.empty_like(c)) for a, b, c in p64]
     plan64 = device.GroupedPlan(p64, "bf16x3", tile=64)
     plan128 = device.GroupedPlan(p128, "bf16x3", tile=128)
-    assert (plan64.tile, plan128.tile) == (64, 128)
+    plan256 = device.GroupedPlan(p256, "bf16x3", tile=256)
+    assert (plan64.tile, plan128.tile, plan256.tile) == (64, 128, 256)
     plan64.run()
     plan128.run()
+    plan256.run()
     torch.cuda.synchronize()
     first = [c.clone() for _, _, c in p64]
-    for (_, _, c64), (_, _, c128) in zip(p64, p128):
+    for (_, _, c64), (_, _, c128), (_, _, c256) in zip(p64, p128, p256):
         assert torch.equal(c64, c128)
+        assert torch.equal(c64, c256)
     for _ in range(3):
         plan64.run()
     torch.cuda.synchronize()
     for c0, (_, _, c) in zip(first, p64):
         assert torch.equal(c0, c)
     _check(p64, TOL["bf16x3"])
+
+
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3"])
+def test_large_tiles(cuda, precision):
+    """The 256-row tiles (256 x 256 for BF16, 256 x 128 for BF16x3) on
+    shapes with partial tiles at every edge."""
+    shapes = [(300, 520, 200), (257, 255, 65), (512, 512, 128), (1, 300, 33), (600, 1, 64)]
+    problems = _problems(shapes, cuda, seed=5)
+    plan = device.GroupedPlan(problems, precision, tile=256)
+    assert plan.tile == 256
+    plan.run()
+    torch.cuda.synchronize()
+    _check(problems, TOL[precision])
+    plan.close()
+
+
+def test_fp32_has_no_large_tile(cuda):
+    """FP32 products asked for 256-row tiles get the autotuner's choice."""
+    problems = _problems([(300, 200, 100)], cuda, seed=6)
+    plan = device.GroupedPlan(problems, "fp32", tile=256)
+    assert plan.tile in (64, 128)
+    plan.run()
+    torch.cuda.synchronize()
+    _check(problems, TOL["fp32"])
+    plan.close()
 
 
 def test_autotuner_picks_a_tile(cuda):
