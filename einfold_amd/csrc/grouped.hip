@@ -250,6 +250,12 @@ void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, int n_items, 
                     for (int i = 0; i < IM; i++)
 #pragma unroll
                         for (int j = 0; j < JN; j++) mma_part<P>(acc[i][j], af[i], bf[j], sp);
+                // All of the chunk's fragment reads before its MFMAs: the
+                // reads' latencies overlap instead of one wait per fragment.
+                if constexpr (P != PREC_F32) {
+                    __builtin_amdgcn_sched_group_barrier(0x100, Elem<P>::planes * (IM + JN), 0);
+                    __builtin_amdgcn_sched_group_barrier(0x008, Parts<P>::n * IM * JN, 0);
+                }
             }
             if (ks + 1 < nk) EF_LSTORE(lds + (cur ^ 1) * BUF);
             __syncthreads();
