@@ -298,9 +298,10 @@ def batched_folds_line(dev):
 
 def large_fold_line(dev):
     """One large dense Float32 fold, 8192 x 8192 x 8192 (2 M N K = 1.1e12
-    FLOP), through the grouped product with 128 x 128 tiles, in each
-    precision: FP32 (FP32 MFMA), BF16x3 (three BF16 products) and BF16.
-    Fractions are of the MFMA peak of the arithmetic issued."""
+    FLOP), through the grouped product with the tile shape its autotuner
+    picks, in each precision: FP32 (FP32 MFMA), BF16x3 (three BF16 products)
+    and BF16.  A run is the operand prep plus the GEMM.  Fractions are of
+    the MFMA peak of the arithmetic issued."""
     import torch
     from einfold_amd import device
     n = 8192
@@ -313,11 +314,11 @@ def large_fold_line(dev):
     rows = torch.arange(0, n, 1024, device=dev)
     want = a[rows].double() @ b.double()
     for precision, issued, peak in (("fp32", 1, PEAK_FP32), ("bf16x3", 3, PEAK_BF16), ("bf16", 1, PEAK_BF16)):
-        plan = device.GroupedPlan([(a, b, c)], precision, 128)
+        plan = device.GroupedPlan([(a, b, c)], precision)
         ms = _events(plan.run, 5)
         err = float(torch.linalg.vector_norm(c[rows].double() - want) / torch.linalg.vector_norm(want))
         tf = flop / (ms * 1e-3) / 1e12
-        line[precision] = {"ms": round(ms, 3), "tflops": round(tf, 1),
+        line[precision] = {"ms": round(ms, 3), "tflops": round(tf, 1), "tile": plan.tile,
                            "frac_of_issued_peak": round(issued * tf / (peak / 1e12), 4),
                            "rel_err_8_rows": err}
         plan.close()
