@@ -1557,6 +1557,226 @@ ef_status prepare(const FoldModel &m, const ef_relation *a, const ef_relation *b
     return EF_OK;
 }
 
+// ---------------------------------------------------------------------
+// A fold of three operands as two dense folds (chain.rs:55-288)
+// ---------------------------------------------------------------------
+
+// chain.rs:57: the largest bound on a partial sum's magnitude taken, 2^1000.
+constexpr double CHAIN_LARGEST_BOUND = 1.0715086071862673e301;
+
+// chain.rs:155-181: every non-NULL factor is a finite Float64, and the
+// product over the operands of their largest magnitude times their rows,
+// which bounds every partial sum in either order, is at most 2^1000.
+bool chain_guarded(const FoldModel &m, const std::vector<const Relation *> &ops) {
+    double bound = 1.0;
+    for (size_t o = 0; o < ops.size(); o++) {
+        const int c = m.operands[o].factor;
+        const Relation &r = *ops[o];
+        if (c < 0 || r.types[c] != Type::Float64) return false;
+        double largest = 0.0;
+        int64_t rows = 0;
+        for (size_t b = 0; b < r.batches.size(); b++) {
+            const ColumnView &v = r.batches[b][c];
+            rows += r.batch_rows[b];
+            const double *x = reinterpret_cast<const double *>(v.values);
+            for (int64_t i = 0; i < v.length; i++) {
+                if (!v.valid(i)) continue;
+                if (!std::isfinite(x[i])) return false;
+                largest = std::max(largest, std::fabs(x[i]));
+            }
+        }
+        bound *= largest * (double)rows;
+    }
+    return bound <= CHAIN_LARGEST_BOUND;
+}
+
+// dense.rs:97-111 for one batch's key column: integers without NULLs, each
+// in [0, 2^63) and below i64::MAX; the extent is the largest plus one.
+bool chain_key_extent(const ColumnView &v, int64_t &extent) {
+    if (!is_integer(v.type) || v.null_count > 0) return false;
+    int64_t hi = -1;
+    for (int64_t i = 0; i < v.length; i++) {
+        int64_t k;
+        switch (v.type) {
+        case Type::Int8: k = reinterpret_cast<const int8_t *>(v.values)[i]; break;
+        case Type::UInt8: k = reinterpret_cast<const uint8_t *>(v.values)[i]; break;
+        case Type::Int16: k = reinterpret_cast<const int16_t *>(v.values)[i]; break;
+        case Type::UInt16: k = reinterpret_cast<const uint16_t *>(v.values)[i]; break;
+        case Type::Int32: k = reinterpret_cast<const int32_t *>(v.values)[i]; break;
+        case Type::UInt32: k = reinterpret_cast<const uint32_t *>(v.values)[i]; break;
+        case Type::Int64: k = reinterpret_cast<const int64_t *>(v.values)[i]; break;
+        default: {
+            const uint64_t u = reinterpret_cast<const uint64_t *>(v.values)[i];
+            if (u >> 63) return false;   // too large for i64
+            k = (int64_t)u;
+        }
+        }
+        if (k < 0) return false;
+        hi = std::max(hi, k);
+    }
+    if (hi == std::numeric_limits<int64_t>::max()) return false;
+    extent = hi + 1;
+    return true;
+}
+
+// chain.rs:197-221: an operand's dimensions in order; the union of two
+// sorted lists.
+std::vector<uint32_t> chain_dims(const OperandModel &op) {
+    std::vector<uint32_t> d;
+    for (auto &k : op.keys) d.push_back(k.first);
+    std::sort(d.begin(), d.end());
+    return d;
+}
+std::vector<uint32_t> chain_union(const std::vector<uint32_t> &a, const std::vector<uint32_t> &b) {
+    std::vector<uint32_t> u(a);
+    u.insert(u.end(), b.begin(), b.end());
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    return u;
+}
+int chain_pos(const std::vector<uint32_t> &local, uint32_t d) {
+    auto it = std::find(local.begin(), local.end(), d);
+    return it == local.end() ? -1 : (int)(it - local.begin());
+}
+// The product of the dimensions' extents; false past 128 bits.
+bool chain_cells(const std::vector<uint32_t> &dims, const std::vector<int64_t> &extent, unsigned __int128 &n) {
+    n = 1;
+    for (uint32_t d : dims) {
+        const unsigned __int128 e = (unsigned __int128)extent[d];
+        if (e != 0 && n > ~(unsigned __int128)0 / e) return false;
+        n *= e;
+    }
+    return true;
+}
+// chain.rs:223-231: of the first pair's dimensions, those the first fold
+// keeps: the outputs and those operand `rest` joins on.
+std::vector<uint32_t> chain_kept(const FoldModel &m, const std::vector<uint32_t> &pair, size_t rest) {
+    std::vector<uint32_t> kept;
+    for (uint32_t d : pair)
+        if (m.output_of(d) >= 0 || m.operands[rest].column_of(d) >= 0) kept.push_back(d);
+    return kept;
+}
+// `op` with its dimensions numbered by their place in `local`.
+OperandModel chain_renumber(const OperandModel &op, const std::vector<uint32_t> &local) {
+    OperandModel r;
+    r.factor = op.factor;
+    for (auto &k : op.keys) r.keys.emplace_back((uint32_t)chain_pos(local, k.first), k.second);
+    return r;
+}
+
+// chain.rs:233-288: the first fold, of operands x and y, keeping one key
+// column per kept dimension, in order, then the sum; the second, of that
+// result and operand `rest`, with the chain's outputs.
+void chain_steps(const FoldModel &m, size_t x, size_t y, size_t rest, FoldModel &first, FoldModel &second) {
+    const std::vector<uint32_t> pair = chain_union(chain_dims(m.operands[x]), chain_dims(m.operands[y]));
+    const std::vector<uint32_t> kept = chain_kept(m, pair, rest);
+    first = FoldModel();
+    for (uint32_t d : pair) first.equality.push_back(m.equality[d]);
+    first.operands = {chain_renumber(m.operands[x], pair), chain_renumber(m.operands[y], pair)};
+    for (uint32_t d : kept) first.outputs.push_back(ef_output{(uint32_t)chain_pos(pair, d), -1, 0});
+    const std::vector<uint32_t> local = chain_union(kept, chain_dims(m.operands[rest]));
+    second = FoldModel();
+    for (uint32_t d : local) second.equality.push_back(m.equality[d]);
+    OperandModel t;
+    for (size_t c = 0; c < kept.size(); c++) t.keys.emplace_back((uint32_t)chain_pos(local, kept[c]), (uint32_t)c);
+    t.factor = (int)kept.size();
+    second.operands = {t, chain_renumber(m.operands[rest], local)};
+    for (auto &o : m.outputs) second.outputs.push_back(ef_output{(uint32_t)chain_pos(local, o.dimension), -1, 0});
+}
+
+// chain.rs:125-153: the dense fold of a and b, or, where it declines, of b
+// and a (the dense algorithm takes holes in its second operand only).
+ef_status chain_either(const FoldModel &f, const Relation &a, const Relation &b, const std::vector<OutField> &fields,
+                       const ef_options *options, std::vector<std::vector<OutColumn>> &out) {
+    {
+        Call c(options);
+        const ef_status st = fold_dense_impl(f, a, b, nullptr, fields, c, out);
+        (void)hipStreamSynchronize(c.stream);
+        if (st != EF_DECLINED) return st;
+    }
+    FoldModel swapped = f;
+    std::swap(swapped.operands[0], swapped.operands[1]);
+    out.clear();
+    Call c(options);
+    const ef_status st = fold_dense_impl(swapped, b, a, nullptr, fields, c, out);
+    (void)hipStreamSynchronize(c.stream);
+    return st;
+}
+
+// The first fold's output batches read as a relation (no copy).
+Relation chain_relation(const std::vector<std::vector<OutColumn>> &batches, const std::vector<OutField> &fields) {
+    Relation r;
+    for (const OutField &f : fields) {
+        r.types.push_back(f.type);
+        r.nullable.push_back(f.nullable);
+        r.formats.push_back(format_of(f.type));
+    }
+    for (const auto &cols : batches) {
+        std::vector<ColumnView> views(cols.size());
+        for (size_t c = 0; c < cols.size(); c++) {
+            ColumnView &v = views[c];
+            v.type = cols[c].type;
+            v.values = cols[c].values ? cols[c].values->data : nullptr;
+            v.validity = cols[c].validity ? cols[c].validity->data : nullptr;
+            v.length = cols[c].length;
+            v.null_count = cols[c].null_count;
+        }
+        const int64_t n = cols.empty() ? 0 : cols[0].length;
+        r.batches.push_back(std::move(views));
+        r.batch_rows.push_back(n);
+        r.rows += n;
+    }
+    return r;
+}
+
+// chain.rs:64-123.
+ef_status fold_chain_impl(const FoldModel &m, const std::vector<const Relation *> &ops,
+                          const std::vector<OutField> &fields, const ef_options *options,
+                          std::vector<std::vector<OutColumn>> &out) {
+    if (!chain_guarded(m, ops)) return EF_DECLINED;
+    // chain.rs:186-198: each dimension's extent over every operand's keys.
+    std::vector<int64_t> extent(m.equality.size(), 0);
+    for (size_t o = 0; o < ops.size(); o++)
+        for (const auto &cols : ops[o]->batches)
+            for (auto &k : m.operands[o].keys) {
+                int64_t e;
+                if (!chain_key_extent(cols[k.second], e)) return EF_DECLINED;
+                extent[k.first] = std::max(extent[k.first], e);
+            }
+    // chain.rs:80-107: the pair contracted first, the cheapest of those that
+    // share a dimension (cells of the pair plus cells of the second fold),
+    // the first of equals.
+    const size_t cand[3][3] = {{0, 1, 2}, {1, 2, 0}, {0, 2, 1}};
+    int best = -1;
+    unsigned __int128 best_cost = 0;
+    for (int i = 0; i < 3; i++) {
+        const size_t x = cand[i][0], y = cand[i][1], rest = cand[i][2];
+        const std::vector<uint32_t> dx = chain_dims(m.operands[x]), dy = chain_dims(m.operands[y]);
+        bool shares = false;
+        for (uint32_t d : dx) shares |= std::find(dy.begin(), dy.end(), d) != dy.end();
+        if (!shares) continue;
+        const std::vector<uint32_t> pair = chain_union(dx, dy);
+        const std::vector<uint32_t> second = chain_union(chain_kept(m, pair, rest), chain_dims(m.operands[rest]));
+        unsigned __int128 c1, c2;
+        if (!chain_cells(pair, extent, c1) || !chain_cells(second, extent, c2) || c1 + c2 < c1) continue;
+        if (best < 0 || c1 + c2 < best_cost) {
+            best = i;
+            best_cost = c1 + c2;
+        }
+    }
+    if (best < 0) return EF_DECLINED;
+    const size_t x = cand[best][0], y = cand[best][1], rest = cand[best][2];
+    FoldModel first, second;
+    chain_steps(m, x, y, rest, first, second);
+    const bool accept32 = options && options->accept_float32;
+    std::vector<OutField> first_fields;
+    if (!fold_schema(first, {ops[x], ops[y]}, nullptr, accept32, first_fields)) return EF_INVALID;
+    std::vector<std::vector<OutColumn>> t;
+    EF_TRY(chain_either(first, *ops[x], *ops[y], first_fields, options, t));
+    const Relation rt = chain_relation(t, first_fields);
+    return chain_either(second, rt, *ops[rest], fields, options, out);
+}
+
 // Runs `body` catching host allocation failures, so no exception crosses
 // the ABI.
 template <class F> ef_status guarded(F body) {
@@ -1661,6 +1881,34 @@ ef_status ef_fold(const ef_fold_desc *fold, const ef_relation *a, const ef_relat
             return st;
         }
         put_result(std::move(batches), fields, EF_ALGORITHM_DISCRIMINATE, out);
+        return EF_OK;
+    });
+}
+
+ef_status ef_fold_chain(const ef_fold_desc *fold, const ef_relation *a, const ef_relation *b, const ef_relation *c,
+                        const ef_options *options, ef_result *out) {
+    clear_error();
+    if (!out) {
+        set_error("no result to write");
+        return EF_INVALID;
+    }
+    std::memset(out, 0, sizeof *out);
+    return guarded([&]() -> ef_status {
+        FoldModel m;
+        if (!parse_fold(fold, m)) return EF_INVALID;
+        // chain.rs:70-78: a SUM of the product of three factors, no domain,
+        // no conditions.
+        if (m.operands.size() != 3 || !m.row.empty() || m.has_domain || !m.conditions.empty()) return EF_DECLINED;
+        for (auto &op : m.operands)
+            if (op.factor < 0) return EF_DECLINED;
+        Relation ra, rb, rc;
+        if (!read_relation(a, ra) || !read_relation(b, rb) || !read_relation(c, rc)) return EF_INVALID;
+        const std::vector<const Relation *> ops = {&ra, &rb, &rc};
+        std::vector<OutField> fields;
+        if (!fold_schema(m, ops, nullptr, options && options->accept_float32, fields)) return EF_INVALID;
+        std::vector<std::vector<OutColumn>> batches;
+        EF_TRY(fold_chain_impl(m, ops, fields, options, batches));
+        put_result(std::move(batches), fields, EF_ALGORITHM_DENSE, out);
         return EF_OK;
     });
 }

@@ -158,6 +158,16 @@ def fold(fold: ir.Fold, a: RelationLike, b: RelationLike, domain: Optional[Relat
     return Folded(ALGORITHMS[r[1].algorithm], r[0])
 
 
+def fold_chain(fold: ir.Fold, a: RelationLike, b: RelationLike, c: RelationLike,
+               options: Optional[Options] = None) -> Optional[List[pa.RecordBatch]]:
+    """algorithm.rs:76-83 (chain.rs:55-288): a SUM of the product of three
+    operands' factors as two dense folds, the cheaper pair first, or None."""
+    keep: list = []
+    desc = ir.fold_to_c(fold, keep)
+    r = _run(_lib.lib().ef_fold_chain, desc, [a, b, c], options)
+    return None if r is None else r[0]
+
+
 def pipeline(p: ir.Pipeline, q: RelationLike, k: RelationLike, v: RelationLike,
              options: Optional[Options] = None) -> Optional[Piped]:
     """algorithm.rs:146-160: the fused pipeline, or None."""

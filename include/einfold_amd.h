@@ -236,6 +236,16 @@ ef_status ef_fold_dense(const ef_fold_desc *fold, const ef_relation *a, const ef
 ef_status ef_fold(const ef_fold_desc *fold, const ef_relation *a, const ef_relation *b,
                   const ef_relation *domain, const ef_options *options, ef_result *out);
 
+/* fold_chain (algorithm.rs:76-83, chain.rs:55-288): a SUM of the product
+ * of three operands' factors, with no domain and no conditions, as two
+ * dense folds, the pair whose two products cost the fewest cells first;
+ * the intermediate stays an owned result between them.  EF_DECLINED
+ * where that doesn't apply: a non-finite factor, sums that could
+ * overflow (the product over the operands of their largest magnitude
+ * times their rows above 2^1000), or keys the dense algorithm declines. */
+ef_status ef_fold_chain(const ef_fold_desc *fold, const ef_relation *a, const ef_relation *b,
+                        const ef_relation *c, const ef_options *options, ef_result *out);
+
 /* pipeline (algorithm.rs:146-160, fused.rs:151-489): the fused tile
  * algorithm with an online softmax and mask tile skipping. */
 ef_status ef_pipeline(const ef_pipeline_desc *pipeline, const ef_relation *q, const ef_relation *k,

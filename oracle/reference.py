@@ -447,6 +447,107 @@ def fold(fold: ir.Fold, a: List[pa.RecordBatch], b: List[pa.RecordBatch],
     return "discriminate", discriminate(fold, ta, tb, domain, out_schema, options)
 
 
+# ----------------------------------------------------------------------
+# chain.rs
+# ----------------------------------------------------------------------
+CHAIN_LARGEST_BOUND = 1.0715086071862673e301   # chain.rs:57: 2**1000
+U128 = 1 << 128
+
+
+def _chain_guarded(fold: ir.Fold, operands) -> bool:
+    """chain.rs:155-181: every non-NULL factor a finite Float64, and the
+    product over the operands of largest magnitude times rows <= 2**1000."""
+    bound = 1.0
+    for op, batches in zip(fold.operands, operands):
+        if op.factor is None:
+            return False
+        largest, rows = 0.0, 0
+        for b in batches:
+            col = b.column(op.factor)
+            if not pa.types.is_float64(col.type):
+                return False
+            rows += b.num_rows
+            for v in col.drop_null().to_pylist():
+                if not np.isfinite(v):
+                    return False
+                largest = max(largest, abs(v))
+        bound *= largest * float(rows)
+    return bound <= CHAIN_LARGEST_BOUND
+
+
+def _chain_steps(fold: ir.Fold, x: int, y: int, rest: int):
+    """chain.rs:233-288: the fold of operands x and y keeping the outputs
+    and the dimensions `rest` joins on, then that result's fold with rest."""
+    ops = fold.operands
+    dims = lambda op: sorted(k.dimension for k in op.keys)
+    pair = sorted(set(dims(ops[x])) | set(dims(ops[y])))
+    kept = [d for d in pair if fold.output_of(d) is not None or ops[rest].column_of(d) is not None]
+    ren = lambda op, local: ir.Operand(tuple(ir.Key(local.index(k.dimension), k.column) for k in op.keys), op.factor)
+    first = ir.Fold([fold.dimensions[d] for d in pair], [ren(ops[x], pair), ren(ops[y], pair)],
+                    [ir.Output(pair.index(d)) for d in kept])
+    local = sorted(set(kept) | set(dims(ops[rest])))
+    t = ir.Operand(tuple(ir.Key(local.index(d), c) for c, d in enumerate(kept)), len(kept))
+    second = ir.Fold([fold.dimensions[d] for d in local], [t, ren(ops[rest], local)],
+                     [ir.Output(local.index(o.dimension)) for o in fold.outputs])
+    return first, second
+
+
+def _chain_either(f: ir.Fold, a, b, out_schema, options):
+    """chain.rs:125-153: dense of (a, b), else of (b, a)."""
+    got = fold_dense(f, a, b, None, out_schema, options)
+    if got is not None:
+        return got
+    swapped = ir.Fold(f.dimensions, [f.operands[1], f.operands[0]], f.outputs)
+    return fold_dense(swapped, b, a, None, out_schema, options)
+
+
+def fold_chain(fold: ir.Fold, operands: List[List[pa.RecordBatch]], schemas: List[pa.Schema],
+               out_schema: pa.Schema, options: Options = Options()):
+    """chain.rs:64-123: a SUM of three factors' product as two dense folds,
+    the cheapest pair sharing a dimension first (first of equals), or None."""
+    chained = (len(fold.operands) == 3 and fold.row is None and fold.domain is None
+               and not fold.conditions and all(op.factor is not None for op in fold.operands))
+    if not chained or not _chain_guarded(fold, operands):
+        return None
+    extent = [0] * len(fold.dimensions)
+    for op, batches in zip(fold.operands, operands):
+        for b in batches:
+            for k in op.keys:
+                r = _keys(b.column(k.column))
+                if r is None:
+                    return None
+                extent[k.dimension] = max(extent[k.dimension], r[1])
+    dims = lambda o: sorted(k.dimension for k in fold.operands[o].keys)
+
+    def cells(ds):   # checked u128 product, in the dimensions' order
+        n = 1
+        for d in ds:
+            n *= extent[d]
+            if n >= U128:
+                return None
+        return n
+    best = None
+    for (x, y), rest in (((0, 1), 2), ((1, 2), 0), ((0, 2), 1)):
+        if not set(dims(x)) & set(dims(y)):
+            continue
+        pair = sorted(set(dims(x)) | set(dims(y)))
+        kept = [d for d in pair if fold.output_of(d) is not None or fold.operands[rest].column_of(d) is not None]
+        c1, c2 = cells(pair), cells(sorted(set(kept) | set(dims(rest))))
+        if c1 is None or c2 is None or c1 + c2 >= U128:
+            continue
+        if best is None or c1 + c2 < best[0]:
+            best = (c1 + c2, x, y, rest)
+    if best is None:
+        return None
+    _, x, y, rest = best
+    first, second = _chain_steps(fold, x, y, rest)
+    first_schema = schema(first, [schemas[x], schemas[y]])
+    t = _chain_either(first, operands[x], operands[y], first_schema, options)
+    if t is None:
+        return None
+    return _chain_either(second, t, operands[rest], out_schema, options)
+
+
 def discriminate(fold: ir.Fold, a: pa.Table, b: pa.Table, domain, out_schema, options: Options):
     """discriminate.rs:369-650."""
     na, nb = a.num_rows, b.num_rows

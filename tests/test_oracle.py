@@ -162,6 +162,164 @@ def test_discriminate_against_nested_loop_sql(seed, strict):
     assert keys == sorted(keys)
 
 
+# ------------------------------------------------------------ chain
+def _grid(n0, n1, seed, salt, names):
+    """Every cell of an n0 x n1 grid, values uniform in [-1, 1)."""
+    i = np.repeat(np.arange(n0), n1)
+    j = np.tile(np.arange(n1), n0)
+    v = data.unit(seed, salt, n0 * n1) * 2 - 1
+    return pa.table({names[0]: i.astype(np.int64), names[1]: j.astype(np.int64), names[2]: v})
+
+
+def chain_fold(shared_ac=False):
+    """A[i, j] B[j, k] C[k, l] -> (i, l); with shared_ac, C[i, k] -> (i)."""
+    if shared_ac:
+        return ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                                  ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2),
+                                  ir.Operand((ir.Key(0, 0), ir.Key(2, 1)), 2)], [ir.Output(0)])
+    return ir.Fold([EQ] * 4, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                              ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2),
+                              ir.Operand((ir.Key(2, 0), ir.Key(3, 1)), 2)], [ir.Output(0), ir.Output(3)])
+
+
+def run_chain(f, tables):
+    sch = R.schema(f, [t.schema for t in tables])
+    return R.fold_chain(f, [t.to_batches() for t in tables], [t.schema for t in tables], sch)
+
+
+def two_folds(first, second, x, y, rest):
+    """The chain's two dense folds in a given order, built by hand."""
+    sch1 = R.schema(first, [x.schema, y.schema])
+    t = R.fold_dense(first, x.to_batches(), y.to_batches(), None, sch1)
+    sch2 = R.schema(second, [sch1, rest.schema])
+    return R.fold_dense(second, t, rest.to_batches(), None, sch2)
+
+
+@pytest.mark.parametrize("ni,nj,nk,nl,order", [(8, 40, 8, 40, "ab"), (40, 8, 40, 8, "bc"), (16, 16, 16, 16, "ab")])
+def test_chain_full_grid_against_numpy_and_order(ni, nj, nk, nl, order):
+    """chain.rs:64-123 on full grids: (i, l) keys exact, sums within the
+    any-order bound of A @ B @ C widened by one rounding per term, and the
+    bits those of the two dense folds in the cheaper order ((A B) C when
+    i j k + i k l is smaller, A (B C) otherwise, the first of equals)."""
+    a, b, c = _grid(ni, nj, 5, 1, "ijv"), _grid(nj, nk, 5, 2, "jkv"), _grid(nk, nl, 5, 3, "klv")
+    f = chain_fold()
+    out = run_chain(f, [a, b, c])
+    t = pa.Table.from_batches(out)
+    # Every (i, l) once; rows come in the second fold's emission order
+    # (its first operand's output dimensions slowest), which SQL leaves open.
+    i, l = t.column(0).to_numpy(), t.column(1).to_numpy()
+    perm = np.lexsort((l, i))
+    assert t.num_rows == ni * nl
+    assert np.array_equal(i[perm], np.repeat(np.arange(ni), nl))
+    assert np.array_equal(l[perm], np.tile(np.arange(nl), ni))
+    A, B, C = (x.column(2).to_numpy().reshape(s) for x, s in ((a, (ni, nj)), (b, (nj, nk)), (c, (nk, nl))))
+    mag = np.abs(A) @ np.abs(B) @ np.abs(C)
+    got = t.column(2).to_numpy()[perm].reshape(ni, nl)
+    assert np.all(np.abs(got - A @ B @ C) <= 2 * (nj * nk + nj + nk) * np.finfo(float).eps * mag)
+    if order == "ab":   # T[i, k] = sum_j A B, then sum_k T C
+        first = ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                                   ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)], [ir.Output(0), ir.Output(2)])
+        second = ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                                    ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)], [ir.Output(0), ir.Output(2)])
+        want = two_folds(first, second, a, b, c)
+    else:               # T[j, l] = sum_k B C, then sum_j A T
+        first = ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                                   ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)], [ir.Output(0), ir.Output(2)])
+        second = ir.Fold([EQ] * 3, [ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2),
+                                    ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2)], [ir.Output(0), ir.Output(2)])
+        want = two_folds(first, second, b, c, a)
+    w = pa.Table.from_batches(want)
+    assert np.array_equal(t.column(2).to_numpy().view(np.uint64), w.column(2).to_numpy().view(np.uint64))
+
+
+def test_chain_shared_dimension_pair_first():
+    """A[i, j] B[j, k] C[i, k] -> (i): with small i, contracting A and C
+    (sharing i) first is cheapest, and the result is sum_jk A B C."""
+    ni, nj, nk = 3, 40, 40
+    a, b, c = _grid(ni, nj, 9, 1, "ijv"), _grid(nj, nk, 9, 2, "jkv"), _grid(ni, nk, 9, 3, "ikv")
+    out = run_chain(chain_fold(shared_ac=True), [a, b, c])
+    t = pa.Table.from_batches(out)
+    A, B, C = a.column(2).to_numpy().reshape(ni, nj), b.column(2).to_numpy().reshape(nj, nk), c.column(2).to_numpy().reshape(ni, nk)
+    want = np.einsum("ij,jk,ik->i", A, B, C)
+    mag = np.einsum("ij,jk,ik->i", np.abs(A), np.abs(B), np.abs(C))
+    assert np.array_equal(t.column(0).to_numpy(), np.arange(ni))
+    assert np.all(np.abs(t.column(1).to_numpy() - want) <= 2 * (nj * nk + nj + nk) * np.finfo(float).eps * mag)
+
+
+def _sparse3(rng, holey, n=(7, 6, 5, 7)):
+    """Three operands over (i, j), (j, k), (k, l), no duplicate key tuples
+    (which the dense algorithm declines).  Operand `holey` has holes and
+    NULL factors; the others are full and NULL-free: each fold's second
+    operand must be (dense.rs:436-467), so one of each pair must be."""
+    def table(n0, n1, names, sparse):
+        rows = [(x, y, None if sparse and rng.random() < 0.15 else float(rng.choice([-2.0, -1.0, 0.5, 1.0, 3.0])))
+                for x in range(n0) for y in range(n1) if not sparse or rng.random() < 0.6]
+        return rows, _table(rows, names, [pa.int64(), pa.int64(), pa.float64()])
+    return (table(n[0], n[1], "ijv", holey == 0), table(n[1], n[2], "jkv", holey == 1),
+            table(n[2], n[3], "klv", holey == 2))
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("holey", [0, 1, 2])
+def test_chain_against_nested_loop_sql(seed, holey):
+    """The three-way join's SQL: groups exist where some (j, k) joins all
+    three, SUM skips NULL products and is NULL when all are; the chain's
+    intermediate T keeps exactly those semantics (chain.rs:20-27)."""
+    rng = np.random.default_rng(seed)
+    (ra, a), (rb, b), (rc, c) = _sparse3(rng, holey)
+    out = run_chain(chain_fold(), [a, b, c])
+    assert out is not None
+    groups = {}
+    for i, j, fa in ra:
+        for j2, k, fb in rb:
+            if j2 != j:
+                continue
+            for k2, l, fc in rc:
+                if k2 != k:
+                    continue
+                s, mag, n = groups.get((i, l), (None, 0.0, 0))
+                if fa is not None and fb is not None and fc is not None:
+                    p = fa * fb * fc
+                    s = p if s is None else s + p
+                    mag += abs(p)
+                    n += 1
+                groups[(i, l)] = (s, mag, n)
+    t = pa.Table.from_batches(out)
+    got = {(i, l): v for i, l, v in zip(t.column(0).to_pylist(), t.column(1).to_pylist(), t.column(2).to_pylist())}
+    assert set(got) == set(groups)
+    for g, (s, mag, n) in groups.items():
+        if s is None:
+            assert got[g] is None, g
+        else:
+            assert got[g] is not None and abs(got[g] - s) <= 2 * (n + 2) * np.finfo(float).eps * mag, (g, got[g], s)
+
+
+def test_chain_declines():
+    """chain.rs:70-78, 155-198: non-finite factors, sums that could
+    overflow, keys the dense algorithm declines, a domain or conditions, no
+    pair sharing a dimension."""
+    a, b, c = _grid(4, 5, 1, 1, "ijv"), _grid(5, 6, 1, 2, "jkv"), _grid(6, 3, 1, 3, "klv")
+    f = chain_fold()
+    assert run_chain(f, [a, b, c]) is not None
+    inf = c.set_column(2, "klv", pa.array(np.where(np.arange(18) == 5, np.inf, c.column(2).to_numpy())))
+    assert run_chain(f, [a, b, inf]) is None
+    # (1e150 * 20 rows)^2 * (|c| < 1) * 18 rows: finite, above 2^1000.
+    big = a.set_column(2, "ijv", pa.array(np.full(20, 1e150)))
+    big_b = big.rename_columns(["j", "k", "v"])
+    assert run_chain(f, [big, big_b, c]) is None
+    assert not R._chain_guarded(f, [big.to_batches(), big_b.to_batches(), c.to_batches()])
+    ok = a.set_column(2, "ijv", pa.array(np.full(20, 1e140)))   # about 7e283: taken
+    assert R._chain_guarded(f, [ok.to_batches(), ok.rename_columns(["j", "k", "v"]).to_batches(), c.to_batches()])
+    neg = b.set_column(0, "j", pa.array(b.column(0).to_numpy() - 1))
+    assert run_chain(f, [a, neg, c]) is None
+    cond = ir.Fold(f.dimensions, f.operands, f.outputs, conditions=(ir.Condition(0, ir.Comparison.LESS, 3, 0),))
+    assert run_chain(cond, [a, b, c]) is None
+    apart = ir.Fold([EQ] * 6, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                               ir.Operand((ir.Key(2, 0), ir.Key(3, 1)), 2),
+                               ir.Operand((ir.Key(4, 0), ir.Key(5, 1)), 2)], [ir.Output(0)])
+    assert run_chain(apart, [a, b, c]) is None
+
+
 def test_ranks_closed_form_and_signed_order():
     """Ranks are positions in numpy.unique, first rows its first occurrences
     (discriminate.rs:280-292); Int64 keys order as signed numbers."""
