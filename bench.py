@@ -73,7 +73,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if args.line:
-        one = {"dense_fold": lambda: dense_fold_line(dev, not args.no_cpu), "ranking": lambda: ranking_line(dev),
+        one = {"dense_fold": lambda: dense_fold_line(dev, not args.no_cpu), "ranking": lambda: ranking_line(dev, not args.no_cpu),
                "batched_folds": lambda: batched_folds_line(dev), "large_fold": lambda: large_fold_line(dev),
                "smoother": smoother_line,
                "regridding": regrid_line}[args.line]
@@ -172,7 +172,7 @@ def main():
         cpu = cpu_baseline()
     secondary = None
     if not args.no_secondary:
-        secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev),
+        secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev, not args.no_cpu),
                      "batched_folds": batched_folds_line(dev), "large_fold": large_fold_line(dev),
                      "smoother": smoother_line(),
                      "regridding": regrid_line()}
@@ -265,6 +265,22 @@ def dense_fold_line(dev, with_cpu: bool):
         dt = (time.perf_counter() - t0) / reps
         line["cpu_port_ms"] = round(dt * 1e3, 3)
         line["cpu_port_cores"] = 1
+        # The host plan the operator replaces (a hash join on j, then a hash
+        # aggregate by (i, k)), stood in for by pyarrow's Acero engine on this
+        # host's cores; it is not DataFusion.
+        import pyarrow as pa
+        import pyarrow.compute as pc
+        plans = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            j = a.join(b, keys="j", join_type="inner", left_suffix="_a", right_suffix="_b")
+            j = j.append_column("p", pc.multiply(j.column("v_a"), j.column("v_b")))
+            g = j.group_by(["i", "k"]).aggregate([("p", "sum")])
+            plans.append(time.perf_counter() - t0)
+        assert g.num_rows == n * n
+        line["hash_plan_ms"] = round(float(np.median(plans)) * 1e3, 1)
+        line["hash_plan"] = (f"pyarrow Acero hash join + hash aggregate ({j.num_rows} joined rows), "
+                             f"{pa.cpu_count()} threads: a stand-in for DataFusion's plan, not DataFusion")
     return line
 
 
@@ -394,10 +410,13 @@ def regrid_line():
             "gflop_per_s": round(2 * fma / (ms * 1e-3) / 1e9, 2)}
 
 
-def ranking_line(dev):
+def ranking_line(dev, with_cpu: bool = False):
     """Radix ranking of the sorting path's join key: j over both operands,
     8M UInt64 hashes of Zipf(1.1) ids over 1M ids.  Algorithmic traffic 12 B
-    per key (read the 8-byte word, write the 4-byte rank)."""
+    per key (read the 8-byte word, write the 4-byte rank).  With the CPU
+    leg, the oracle's C restatement of discriminate.rs's ranking on one
+    core over the same keys, whose ranks and first rows must equal the
+    device's bit for bit."""
     import ctypes
     import torch
     from einfold_amd import data
@@ -416,9 +435,23 @@ def ranking_line(dev):
     run = lambda: check(lib().ef_rank_u64(P(words), n, P(rank), P(first), ctypes.byref(k), P(scratch), stream))
     ms = _events(run, 10)
     gbs = 12.0 * n / (ms * 1e-3) / 1e9
-    return {"workload": "radix ranking of 8M UInt64 join keys (Zipf 1.1 over 1M ids)", "keys": n,
+    line = {"workload": "radix ranking of 8M UInt64 join keys (Zipf 1.1 over 1M ids)", "keys": n,
             "distinct": int(k.value), "ms": round(ms, 4), "gb_per_s": round(gbs, 1),
             "frac_of_hbm_peak": round(gbs / 8000.0, 4)}
+    if with_cpu:
+        import oracle
+        host = np.concatenate([a_j, b_j])
+        t0 = time.perf_counter()
+        want_rank, want_first = oracle.ranks_of_words(host)
+        line["cpu_port_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        line["cpu_port_cores"] = 1
+        torch.cuda.synchronize()
+        same = (np.array_equal(rank.cpu().numpy().view(np.uint32), want_rank) and
+                np.array_equal(first[:k.value].cpu().numpy().view(np.uint32), want_first))
+        if not same:
+            raise SystemExit("ranking: device ranks differ from the oracle's")
+        line["bit_exact_vs_oracle"] = True
+    return line
 
 
 def cpu_baseline():
