@@ -56,7 +56,7 @@ def main():
                     help="print one secondary line alone (for profiling one workload) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})",
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
 

@@ -8,7 +8,7 @@
 * the left-joined gradient folds that ddx writes for dV
   (docs/spikes/s31-ddx-gradient-steps): dV(h, tk, d) = SUM over tq of
   P(h, tq, tk) * dO(h, tq, d), left-joined from the v table with 0.0 for
-  keys no query reaches, run by this build's dense fold operator.
+  keys no query reaches, run by the dense fold operator (ef_fold_dense).
 """
 import numpy as np
 import pyarrow as pa
