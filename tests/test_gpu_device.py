@@ -239,3 +239,44 @@ def test_attention_backward_against_float64_autograd(cuda, causal, precision, to
         w = want.numpy()
         err = np.linalg.norm(got.double().cpu().numpy() - w) / np.linalg.norm(w)
         assert err < tol, err
+
+
+def _allowed_spans(T, seed):
+    """Per-query key ranges [lo, hi), each non-empty, of varying width."""
+    rng = np.random.default_rng(seed)
+    lo = rng.integers(0, T - 1, T)
+    hi = np.minimum(T, lo + 1 + rng.integers(0, 150, T))
+    return lo.astype(np.int64), hi.astype(np.int64)
+
+
+@pytest.mark.parametrize("mask", ["window", "allowed"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("fp32", 1e-5), ("bf16x3", 3e-5)])
+def test_attention_backward_masks_against_float64_autograd(cuda, mask, precision, tol):
+    """The masked tile paths of the backward kernels (a sliding window
+    under the causal mask, and arbitrary per-query key ranges) against
+    torch's float64 autograd of the same masked softmax, at a ragged T."""
+    print(f"attention bwd mask={mask} {precision}", flush=True)
+    B, T, D = 2, 200, 64
+    q, k, v = data.attention_tables(B, T, D, seed=8)
+    g = data.normal(8, 9, B * T * D).reshape(B, T, D)
+    scale = 0.125
+    if mask == "window":
+        lo, hi = _spans(T, T, True, 37)
+        kw = dict(causal=True, window=37)
+    else:
+        lo, hi = _allowed_spans(T, 5)
+        kw = dict(allowed=torch.from_numpy(np.stack([lo, hi], 1)).to(cuda))
+    keys = np.arange(T)
+    blocked = torch.from_numpy(~((keys[None, :] >= lo[:, None]) & (keys[None, :] < hi[:, None])))
+    qt, kt, vt = (torch.from_numpy(x).requires_grad_() for x in (q, k, v))
+    s = ((qt @ kt.transpose(1, 2)) * scale).masked_fill(blocked, float("-inf"))
+    o = torch.softmax(s, -1) @ vt
+    (o * torch.from_numpy(g)).sum().backward()
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dt).to(cuda)
+    out, m, l = device.attention_fwd(t(q), t(k), t(v), scale, precision=precision, **kw)
+    dq, dk, dv = device.attention_bwd(t(q), t(k), t(v), out, t(g), m, l, scale, precision=precision, **kw)
+    for got, want in ((dq, qt.grad), (dk, kt.grad), (dv, vt.grad)):
+        w = want.numpy()
+        err = np.linalg.norm(got.double().cpu().numpy() - w) / np.linalg.norm(w)
+        assert err < tol, err
