@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 256 MiB device copy first (a known-size kernel for counter scaling)")
     ap.add_argument("--line", choices=["dense_fold", "ranking", "batched_folds", "large_fold", "smoother",
-                                       "regridding"],
+                                       "regridding", "chain"],
                     help="print one secondary line alone (for profiling one workload) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -76,7 +76,7 @@ def main():
         one = {"dense_fold": lambda: dense_fold_line(dev, not args.no_cpu), "ranking": lambda: ranking_line(dev, not args.no_cpu),
                "batched_folds": lambda: batched_folds_line(dev), "large_fold": lambda: large_fold_line(dev),
                "smoother": smoother_line,
-               "regridding": regrid_line}[args.line]
+               "regridding": regrid_line, "chain": lambda: chain_line(not args.no_cpu)}[args.line]
         print(json.dumps({args.line: one()}), flush=True)
         return
     q64, k64, v64 = data.attention_tables(H, T, D, seed=3)
@@ -175,7 +175,7 @@ def main():
         secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev, not args.no_cpu),
                      "batched_folds": batched_folds_line(dev), "large_fold": large_fold_line(dev),
                      "smoother": smoother_line(),
-                     "regridding": regrid_line()}
+                     "regridding": regrid_line(), "chain": chain_line(not args.no_cpu)}
 
     line = {
         "metric": "fold GFLOP/s + attention fwd+bwd ms: mixed vs plain FP32, 1 MI355X",
@@ -221,6 +221,55 @@ def _events(fn, reps, stream_fn=None):
     b.record()
     torch.cuda.synchronize()
     return a.elapsed_time(b) / reps
+
+
+def chain_line(with_cpu: bool):
+    """A chain of three dense products over Arrow (kernel API fold_chain):
+    A[i, j] B[j, k] C[k, l] with i = k = 256, j = l = 1024, every cell, FP64
+    values U[-1, 1]; (A B) C is cheaper, 2 * 2 * 256^2 * 1024 FLOP in the
+    two dense contractions (exact reference order, FP64 VALU).  Arrow to
+    Arrow on the device, and the oracle's chain (the same two folds, its C
+    contraction) on one core, whose result the device's must equal bit for
+    bit."""
+    import pyarrow as pa
+    from einfold_amd import data, ir, kernels
+
+    def grid(n0, n1, salt, names):
+        v = data.unit(17, salt, n0 * n1) * 2 - 1
+        return pa.table({names[0]: np.repeat(np.arange(n0), n1), names[1]: np.tile(np.arange(n1), n0),
+                         names[2]: v})
+    ni, nj, nk, nl = 256, 1024, 256, 1024
+    a, b, c = grid(ni, nj, 1, "ijv"), grid(nj, nk, 2, "jkv"), grid(nk, nl, 3, "klv")
+    EQ = ir.KeyEquality.EQUAL
+    fold = ir.Fold([EQ] * 4, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                              ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2),
+                              ir.Operand((ir.Key(2, 0), ir.Key(3, 1)), 2)], [ir.Output(0), ir.Output(3)])
+    out = kernels.fold_chain(fold, a, b, c)
+    walls = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        out = kernels.fold_chain(fold, a, b, c)
+        walls.append(time.perf_counter() - t0)
+    assert out is not None and sum(x.num_rows for x in out) == ni * nl
+    flop = 2.0 * ni * nj * nk + 2.0 * ni * nk * nl
+    ms = float(np.median(walls)) * 1e3
+    line = {"workload": "chain A[256x1024] B[1024x256] C[256x1024] over Arrow (fold_chain), FP64, "
+                        "(A B) C in exact dense order", "gflop": round(flop / 1e9, 3),
+            "arrow_to_arrow_ms": round(ms, 2), "gflop_per_s_arrow_to_arrow": round(flop / (ms * 1e-3) / 1e9, 1)}
+    if with_cpu:
+        from oracle import reference as R
+        sch = R.schema(fold, [a.schema, b.schema, c.schema])
+        t0 = time.perf_counter()
+        want = R.fold_chain(fold, [a.to_batches(), b.to_batches(), c.to_batches()],
+                            [a.schema, b.schema, c.schema], sch)
+        line["cpu_port_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        line["cpu_port_cores"] = 1
+        got_v = pa.Table.from_batches(out).column(2).to_numpy()
+        want_v = pa.Table.from_batches(want).column(2).to_numpy()
+        if not np.array_equal(got_v.view(np.uint64), want_v.view(np.uint64)):
+            raise SystemExit("chain: device sums differ from the oracle's")
+        line["bit_exact_vs_oracle"] = True
+    return line
 
 
 def dense_fold_line(dev, with_cpu: bool):

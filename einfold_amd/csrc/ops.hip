@@ -16,9 +16,16 @@ namespace {
 
 // ---- keys ----------------------------------------------------------
 
+// Keys cast to i64, with their minimum, maximum and whether any unsigned
+// key is 2^63 or more.  The bounds are reduced over the wave, then over the
+// workgroup's four waves in LDS, so each workgroup adds one atomic per
+// bound: atomics on one address serialize (one per wave cost 78 us over
+// 262k keys).
 template <class T>
 __global__ __launch_bounds__(256) void cast_keys_kernel(const T *__restrict__ in, int64_t n,
                                                         int64_t *__restrict__ out, KeyStats *stats) {
+    __shared__ int64_t slo[4], shi[4];
+    __shared__ int sbad[4];
     int64_t lo = 0, hi = -1;
     int bad = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -41,7 +48,19 @@ __global__ __launch_bounds__(256) void cast_keys_kernel(const T *__restrict__ in
         hi = h2 > hi ? h2 : hi;
         bad |= __shfl_xor(bad, o);
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        slo[w] = lo;
+        shi[w] = hi;
+        sbad[w] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 4; i++) {
+            lo = slo[i] < lo ? slo[i] : lo;
+            hi = shi[i] > hi ? shi[i] : hi;
+            bad |= sbad[i];
+        }
         atomicMin((long long *)&stats->lo, (long long)lo);
         atomicMax((long long *)&stats->hi, (long long)hi);
         if (bad) atomicOr(&stats->bad, 1);
@@ -257,7 +276,7 @@ __global__ __launch_bounds__(256) void widen_keys_kernel(int64_t n, const T *__r
 
 ef_status cast_keys(Type t, const void *in, int64_t n, int64_t *out, KeyStats *stats, hipStream_t s) {
     if (n == 0) return EF_OK;
-    const unsigned g = blocks_for(n, 256) > 1024 ? 1024 : blocks_for(n, 256);
+    const unsigned g = blocks_for(n, 256) > 256 ? 256 : blocks_for(n, 256);   // at most 256 atomics per bound
 #define EF_CAST(T) hipLaunchKernelGGL(cast_keys_kernel<T>, dim3(g), dim3(256), 0, s, (const T *)in, n, out, stats)
     switch (t) {
     case Type::Int8: EF_CAST(int8_t); break;
