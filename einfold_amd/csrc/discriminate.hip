@@ -117,9 +117,12 @@ __global__ __launch_bounds__(256) void valid_word_kernel(const uint32_t *rows, i
     if (i < n) out[i] = valid[rows[i]] ? 1ULL : 0ULL;
 }
 
+// Whether any sorted word repeats its predecessor.  One atomic per wave at
+// most: atomics on one address serialize, and repeats can be most rows.
 __global__ __launch_bounds__(256) void repeated_kernel(const uint64_t *w, int64_t n, int *any) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > 0 && i < n && w[i] == w[i - 1]) atomicOr(any, 1);
+    const bool rep = i > 0 && i < n && w[i] == w[i - 1];
+    if (__any(rep) && (threadIdx.x & 63) == 0) atomicOr(any, 1);
 }
 
 // start[k] = the first sorted index whose key (w >> 32) is at least k, for
