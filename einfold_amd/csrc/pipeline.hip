@@ -130,7 +130,9 @@ __device__ double eval(const ef_expr_op *e, int n, const double *qv, const doubl
             case EF_EXPR_SUB: r = a - b; break;
             case EF_EXPR_MUL: r = a * b; break;
             case EF_EXPR_DIV: r = a / b; break;
-            default: r = pow(a, b); break;
+            // A square as one correctly rounded product (within pow's own
+            // error bound), not the general pow's logarithm and exponential.
+            default: r = b == 2.0 ? a * a : pow(a, b); break;
             }
             st[sp - 1] = r;
         }
