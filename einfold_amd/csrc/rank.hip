@@ -391,9 +391,12 @@ struct AssignArgs {
 // LDS by bucket of rows (in any order inside a bucket) and written as runs,
 // for rows_from_buckets_kernel: 4-byte stores to random rows cost HBM a
 // 32-byte sector each.
+constexpr int PARTS = KPT * WAVES;
+static_assert(PARTS <= 64, "one wave scans the start counts of a tile's (round, wave) groups");
+
 template <bool BUCKETS>
 __global__ __launch_bounds__(THREADS) void assign_kernel(AssignArgs a) {
-    __shared__ uint32_t part[KPT * WAVES];   // starts per (round, wave), then their exclusive prefix
+    __shared__ uint32_t part[PARTS];   // starts per (round, wave), then their exclusive prefix
     __shared__ uint64_t sp[BUCKETS ? TILE : 1];   // staged pairs
     __shared__ uint32_t bc[BUCKETS ? 256 : 1];    // pairs of each bucket, then the bucket's LDS start
     __shared__ uint32_t bdst[BUCKETS ? 256 : 1];  // output position of a bucket's run, less its LDS start
@@ -438,9 +441,11 @@ __global__ __launch_bounds__(THREADS) void assign_kernel(AssignArgs a) {
         if (lane == 63) bwt[wv] = bincl;
     }
     if (wv == 0) {
-        const uint32_t v = part[lane];
+        // One wave scans the KPT * WAVES counts; lanes past them add 0 and
+        // leave the neighbouring LDS arrays alone.
+        const uint32_t v = lane < PARTS ? part[lane] : 0u;
         const uint32_t incl = wave_inclusive_sum(v, lane);
-        part[lane] = incl - v;
+        if (lane < PARTS) part[lane] = incl - v;
         if (lane == 63 && tile == a.tiles - 1) *a.n_ranks = before + incl;
     }
     __syncthreads();
