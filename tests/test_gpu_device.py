@@ -112,6 +112,17 @@ def test_ranks_zipf_keys(cuda):
     _check_ranks(data.hash64(ids, 0x6A), cuda)
 
 
+def test_ranks_full_size_join_keys(cuda):
+    """The 8M-key workload: join keys of two 4M-row operands (Zipf 1.1 over
+    1M ids), concatenated as `ranks_of_words` takes them
+    (discriminate.rs:73-97); every byte varies, so all eight passes run."""
+    print("ranks 8M join keys", flush=True)
+    rows = 4 << 20
+    a = data.hash64(data.zipf_ids(5, 1, rows, 1 << 20, 1.1), 0x6A)
+    b = data.hash64(data.zipf_ids(5, 2, rows, 1 << 20, 1.1), 0x6A)
+    _check_ranks(np.concatenate([a, b]), cuda)
+
+
 # ---------------------------------------------------------------- attention
 def _np_attention(q, k, v, scale, lo, hi):
     """float64 masked softmax attention, the NumpyBackend formula
