@@ -69,6 +69,35 @@ def fold_rows(a_rows, b_rows, a_dims, b_dims, outputs, strict, conditions=(),
     return {v[3]: (v[0], v[1], v[2]) for v in groups.values()}
 
 
+def _matches(d, g, equality, signed_zeros_equal):
+    """A domain key against a group key: `=` (equality 0) never matches
+    NULL; IS NOT DISTINCT FROM (1) matches NULL with NULL."""
+    if d is None or g is None:
+        return equality == 1 and d is None and g is None
+    if isinstance(d, float) or isinstance(g, float):
+        return _key(float(d), signed_zeros_equal) == _key(float(g), signed_zeros_equal)
+    return int(d) == int(g)
+
+
+def domain_left_join(groups, domain_keys, equalities, missing, signed_zeros_equal=False):
+    """SELECT COALESCE(folded.value, missing) FROM domain LEFT JOIN folded
+    ON domain.key_i <equality_i> folded.output_i (fold.rs:311-339).
+
+    groups: fold_rows' result; domain_keys: one tuple per domain row, in the
+    outputs' order (None for NULL).  Returns, per domain row in order, the
+    list of (value, sum of |terms|, n) it gets: one entry per group its keys
+    match, or one (missing, 0, 0) entry when none does.  A NULL group value
+    becomes `missing` too (COALESCE)."""
+    out = []
+    for dk in domain_keys:
+        hits = []
+        for gk, (s, mag, n) in groups.items():
+            if all(_matches(d, g, e, signed_zeros_equal) for d, g, e in zip(dk, gk, equalities)):
+                hits.append((missing, 0.0, 0) if s is None else (s, mag, n))
+        out.append(hits if hits else [(missing, 0.0, 0)])
+    return out
+
+
 def _holds(l, op, r, off):
     if l is None or r is None:
         return False
