@@ -96,8 +96,8 @@ def main():
         return desc
 
     def buffers(qx, dtype):
-        return dict(out=torch.empty_like(qx), m=torch.empty((H, T), dtype=torch.float32, device=dev),
-                    l=torch.empty((H, T), dtype=torch.float32, device=dev),
+        return dict(out=torch.empty_like(qx), m=torch.empty((H, T), dtype=dtype, device=dev),
+                    l=torch.empty((H, T), dtype=dtype, device=dev),
                     dq=torch.empty_like(qx), dk=torch.empty_like(qx), dv=torch.empty_like(qx))
 
     P = lambda t: ctypes.c_void_p(t.data_ptr())

@@ -74,7 +74,9 @@ class Mi355xBackend:
             out = np.empty(w * x * y, np.float64)
             status = L.ef_dense_run_attention(P(q), P(kt), P(v), _lib.CAttentionShape(w, x, e, j, y), int(code),
                                               float(c), P(spans), P(out), None)
-        if status == _lib.DECLINED:
+        # A decline, or device memory running short: einfold computes the
+        # work itself (its host plan), as for any backend that says no.
+        if status in (_lib.DECLINED, _lib.OUT_OF_MEMORY):
             return None
         _lib.check(status)
         return pa.array(out, type=pa.float64())

@@ -109,7 +109,7 @@ size_t ef_attention_scratch_bytes(const ef_attention_desc *desc) {
 }
 
 ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
-                           void *out, float *row_max, float *row_sum, uint64_t *tiles_visited,
+                           void *out, void *row_max, void *row_sum, uint64_t *tiles_visited,
                            void *scratch, void *stream) {
     clear_error();
     return with_scratch(desc, scratch, stream, [&](void *sc) {
@@ -118,7 +118,7 @@ ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const v
 }
 
 ef_status ef_attention_bwd(const ef_attention_desc *desc, const void *q, const void *k, const void *v,
-                           const void *out, const void *dout, const float *row_max, const float *row_sum,
+                           const void *out, const void *dout, const void *row_max, const void *row_sum,
                            void *dq, void *dk, void *dv, void *scratch, void *stream) {
     clear_error();
     return with_scratch(desc, scratch, stream, [&](void *sc) {
@@ -137,8 +137,8 @@ ef_status ef_attention_bwd_prepare(const ef_attention_desc *desc, const void *q,
                          as_stream(stream));
 }
 
-ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *k, const void *v, const float *row_max,
-                              const float *row_sum, void *scratch, void *dk, void *dv, void *stream) {
+ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *k, const void *v, const void *row_max,
+                              const void *row_sum, void *scratch, void *dk, void *dv, void *stream) {
     clear_error();
     if (!desc || !scratch) {
         set_error("a description and the prepared scratch are required");
@@ -148,8 +148,8 @@ ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *k, cons
                          as_stream(stream));
 }
 
-ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const void *dout, const float *row_max,
-                             const float *row_sum, void *scratch, void *dq, void *stream) {
+ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const void *dout, const void *row_max,
+                             const void *row_sum, void *scratch, void *dq, void *stream) {
     clear_error();
     if (!desc || !scratch) {
         set_error("a description and the prepared scratch are required");
@@ -236,8 +236,8 @@ ef_status ef_dense_run_attention(const double *q, const double *kt, const double
     EF_TRY(dk.alloc(B * J * E * 8));
     EF_TRY(dv.alloc(B * J * Y * 8));
     EF_TRY(dout.alloc(no * 8));
-    EF_TRY(dm.alloc(B * X * 4));
-    EF_TRY(dl.alloc(B * X * 4));
+    EF_TRY(dm.alloc(B * X * 8));
+    EF_TRY(dl.alloc(B * X * 8));
     EF_TRY(dsp.alloc(2 * X * 8));
     if (B * X * E) EF_HIP(hipMemcpyAsync(dq.ptr, q, B * X * E * 8, hipMemcpyHostToDevice, s));
     if (B * J * E) EF_HIP(hipMemcpyAsync(dk.ptr, k.data(), B * J * E * 8, hipMemcpyHostToDevice, s));
@@ -246,12 +246,15 @@ ef_status ef_dense_run_attention(const double *q, const double *kt, const double
     ef_attention_desc d;
     std::memset(&d, 0, sizeof d);
     d.batches = B; d.queries = X; d.keys = J; d.features = E; d.values = Y;
+    // Scale::Over(c) divides each score by c (backend.rs:49-86); the kernels
+    // multiply by the rounded 1/c, one more rounding per score, which the
+    // backend contract's tolerance absorbs (its bits are not pinned).
     d.scale = scale_kind == EF_SCALE_TIMES ? scale : (scale_kind == EF_SCALE_OVER ? 1.0 / scale : 1.0);
     d.allowed = dsp.as<int64_t>();
     d.precision = EF_PRECISION_FP64;
     DeviceBuffer scratch;
     EF_TRY(scratch.alloc(attention_scratch_bytes(d)));
-    EF_TRY(attention_fwd(d, dq.ptr, dk.ptr, dv.ptr, dout.ptr, dm.as<float>(), dl.as<float>(), nullptr, scratch.ptr, s));
+    EF_TRY(attention_fwd(d, dq.ptr, dk.ptr, dv.ptr, dout.ptr, dm.ptr, dl.ptr, nullptr, scratch.ptr, s));
     EF_HIP(hipMemcpyAsync(out, dout.ptr, no * 8, hipMemcpyDeviceToHost, s));
     EF_HIP(hipStreamSynchronize(s));
     return EF_OK;

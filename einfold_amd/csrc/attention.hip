@@ -45,6 +45,14 @@ constexpr double LN2 = 0.6931471805599453;
 constexpr double TAU = 8.0;     // forward weights stay below 2^TAU (log2 units)
 
 struct Span { int64_t lo, hi; };
+
+// Per-row statistics (row max, row sum, delta): float for the FP32 and
+// BF16 precisions, double for FP64, so that the FP64 backward recomputes
+// its weights from FP64 statistics.
+template <int P> struct Stat { typedef float T; };
+template <> struct Stat<PREC_F64> { typedef double T; };
+__device__ __forceinline__ float log2_of(float x) { return __log2f(x); }
+__device__ __forceinline__ double log2_of(double x) { return log2(x); }
 struct Span32 { int lo, hi; };
 
 struct Mask {
@@ -115,7 +123,7 @@ struct PrepArgs {
     int64_t B;
     Counter *zero;
     const void *out, *dout;
-    float *delta;
+    void *delta;                  // Stat<P>::T per query
     int64_t tq;
 };
 
@@ -132,16 +140,17 @@ __global__ __launch_bounds__(256) void prep_kernel(PrepArgs a) {
     const int64_t t0 = (int64_t)blockIdx.x * 64;
     if ((int)blockIdx.z == a.njobs) {
         // delta rows t0 .. t0 + 63 of batch b
+        typedef typename Stat<P>::T St;
         const In *out = static_cast<const In *>(a.out), *dout = static_cast<const In *>(a.dout);
         const int c = threadIdx.x & 15;
         for (int it = 0; it < 4; it++) {
             const int64_t x = t0 + it * 16 + (threadIdx.x >> 4);
             const int64_t r = b * a.tq + x;
-            float sm = 0;
+            St sm = 0;
             if (x < a.tq)
-                for (int e = c; e < D; e += 16) sm += (float)out[r * D + e] * (float)dout[r * D + e];
+                for (int e = c; e < D; e += 16) sm += (St)out[r * D + e] * (St)dout[r * D + e];
             for (int off = 8; off >= 1; off >>= 1) sm += __shfl_xor(sm, off, 16);
-            if (x < a.tq && c == 0) a.delta[r] = sm;
+            if (x < a.tq && c == 0) static_cast<St *>(a.delta)[r] = sm;
         }
         return;
     }
@@ -282,7 +291,7 @@ struct FwdArgs {
     const void *q;
     const void *kr, *vt;          // prepared K rows and V^T
     void *out;
-    float *row_max, *row_sum;
+    void *row_max, *row_sum;      // Stat<P>::T per query
     unsigned long long *visited;
     Mask mask;
     int64_t tq, tk, tkp, batches;
@@ -616,8 +625,9 @@ void attn_fwd_kernel(FwdArgs a) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) O[db * 16 + crow<P>(g, i)] = (In)(o[qb][db][i] / l[qb]);
             if (g == 0) {
-                a.row_max[b * a.tq + x[qb]] = (float)(m[qb] * (S)LN2);
-                a.row_sum[b * a.tq + x[qb]] = (float)l[qb];
+                typedef typename Stat<P>::T St;
+                static_cast<St *>(a.row_max)[b * a.tq + x[qb]] = (St)(m[qb] * (S)LN2);
+                static_cast<St *>(a.row_sum)[b * a.tq + x[qb]] = (St)l[qb];
             }
         }
     }
@@ -630,7 +640,7 @@ void attn_fwd_kernel(FwdArgs a) {
 // sums rescaled to the larger max (log2 units), then normalized.
 template <class S, class In>
 __global__ __launch_bounds__(256) void attn_merge_kernel(const S *po, const S *pm, const S *pl, int64_t rows, int d,
-                                                         In *out, float *row_max, float *row_sum) {
+                                                         In *out, S *row_max, S *row_sum) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows * d) return;
     const int64_t r = i / d;
@@ -641,8 +651,8 @@ __global__ __launch_bounds__(256) void attn_merge_kernel(const S *po, const S *p
     const S l = pl[r] * a0 + pl[rows + r] * a1;
     out[i] = (In)((po[i] * a0 + po[rows * d + i] * a1) / l);
     if (i % d == 0) {
-        row_max[r] = (float)(mx * (S)LN2);
-        row_sum[r] = (float)l;
+        row_max[r] = mx * (S)LN2;
+        row_sum[r] = l;
     }
 }
 
@@ -666,7 +676,7 @@ __global__ __launch_bounds__(256) void attn_sum_kernel(const S *p, int64_t n, do
 // Float32 arithmetic throughout.
 template <class T>
 __global__ __launch_bounds__(256) void attn_fwd_generic(const T *Q, const T *K, const T *V, T *O,
-                                                         float *row_max, float *row_sum, Mask mask,
+                                                         T *row_max, T *row_sum, Mask mask,
                                                          int64_t batches, int64_t tq, int64_t tk,
                                                          int64_t d, int64_t dv, double scale) {
     constexpr int YC = 8;
@@ -703,8 +713,8 @@ __global__ __launch_bounds__(256) void attn_fwd_generic(const T *Q, const T *K, 
         if (y < dv) O[(b * tq + x) * dv + y] = acc[i] / z;
     }
     if (yc == 0) {
-        row_max[b * tq + x] = (float)m;
-        row_sum[b * tq + x] = (float)z;
+        row_max[b * tq + x] = m;
+        row_sum[b * tq + x] = z;
     }
 }
 
@@ -716,7 +726,7 @@ struct BwdArgs {
     const void *dout;
     const void *qr, *qt, *dor, *dot;   // prepared query-side tiles
     const void *kr, *kt, *vr;          // prepared key-side tiles
-    const float *row_max, *row_sum, *delta;
+    const void *row_max, *row_sum, *delta;   // Stat<P>::T per query
     void *dq, *dk, *dv;
     Counter *ctr;
     Mask mask;
@@ -743,7 +753,8 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     __shared__ __attribute__((aligned(16))) E qtr[PL * TPLANE];
     __shared__ __attribute__((aligned(16))) E dos[PL * RPLANE];
     __shared__ __attribute__((aligned(16))) E dotr[PL * TPLANE];
-    __shared__ float sm[BN], sd[BN];   // per query: m log2(e) + log2(l), and delta
+    typedef typename Stat<P>::T St;
+    __shared__ St sm[BN], sd[BN];   // per query: m log2(e) + log2(l), and delta
     __shared__ int64_t slo[BN], shi[BN];
     __shared__ int slot;
 
@@ -771,7 +782,9 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         const E *QT = static_cast<const E *>(a.qt) + b * D * a.tqp;
         const E *DR = static_cast<const E *>(a.dor) + b * a.tqp * D;
         const E *DT = static_cast<const E *>(a.dot) + b * D * a.tqp;
-        const float *RM = a.row_max + b * a.tq, *RS = a.row_sum + b * a.tq, *DL = a.delta + b * a.tq;
+        const St *RM = static_cast<const St *>(a.row_max) + b * a.tq;
+        const St *RS = static_cast<const St *>(a.row_sum) + b * a.tq;
+        const St *DL = static_cast<const St *>(a.delta) + b * a.tq;
 
         int64_t kj[2];
         Frag<P> kf[2][KC], vf[2][KC];
@@ -820,7 +833,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         typedef TileIO<E, PL, D, BN> ColT;   // [feature][query]
         constexpr bool PF = 2 * (RowT::N + ColT::N) <= 16;
         u32x4 pq[RowT::N], pqt[ColT::N], pd[RowT::N], pdt[ColT::N];
-        float prm = 0.f, prs = 1.f, pdl = 0.f;
+        St prm = 0, prs = 1, pdl = 0;
 #define EF_FETCH(T_)                                                                         \
     do {                                                                                     \
         const int64_t q0_ = (T_) * BN;                                                       \
@@ -857,8 +870,8 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                 if (tid < BN) {
                     slo[tid] = sp.lo;
                     shi[tid] = sp.hi;
-                    sm[tid] = xq < a.tq ? prm * (float)LOG2E + __log2f(prs) : 0.f;
-                    sd[tid] = xq < a.tq ? pdl : 0.f;
+                    sm[tid] = xq < a.tq ? prm * (St)LOG2E + log2_of(prs) : St(0);
+                    sd[tid] = xq < a.tq ? pdl : St(0);
                 }
             }
             __syncthreads();
@@ -1059,9 +1072,11 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
             }
             // m log2(e) + log2(l): P = exp2(scale s log2(e) - mrow).
             {
+                typedef typename Stat<P>::T St;
                 const int64_t xr = b * a.tq + (x[qb] < a.tq ? x[qb] : a.tq - 1);
-                const float rm = a.row_max[xr], rsum = a.row_sum[xr], dl = a.delta[xr];
-                mrow[qb] = x[qb] < a.tq ? (S)rm * (S)LOG2E + (S)__log2f(rsum) : S(0);
+                const St rm = static_cast<const St *>(a.row_max)[xr], rsum = static_cast<const St *>(a.row_sum)[xr];
+                const St dl = static_cast<const St *>(a.delta)[xr];
+                mrow[qb] = x[qb] < a.tq ? (S)(rm * (St)LOG2E + log2_of(rsum)) : S(0);
                 drow[qb] = x[qb] < a.tq ? (S)dl : S(0);
             }
         }
@@ -1281,7 +1296,7 @@ Layout layout(const ef_attention_desc &d) {
     L.bkr = at; at += ka;
     L.bkt = at; at += ka;
     L.bvr = at; at += ka;
-    L.delta = at; at += align256(d.batches * d.queries * 4);
+    L.delta = at; at += align256(d.batches * d.queries * (d.precision == EF_PRECISION_FP64 ? 8 : 4));
     at = at > fwd_end ? at : fwd_end;
     const size_t sb = d.precision == EF_PRECISION_FP64 ? 8 : 4;
     const size_t rq = 2 * d.batches * d.queries, rk = 2 * d.batches * d.keys;
@@ -1297,7 +1312,7 @@ Layout layout(const ef_attention_desc &d) {
 
 template <int P, int D>
 ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, void *out,
-                  float *row_max, float *row_sum, uint64_t *visited, char *scratch, hipStream_t s) {
+                  void *row_max, void *row_sum, uint64_t *visited, char *scratch, hipStream_t s) {
     typedef typename Input<P>::T In;
     typedef typename Elem<P>::T E;
     const Layout L = layout(d);
@@ -1334,7 +1349,8 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
         typedef typename Acc<P>::scalar S;
         const int64_t rows = B * tq;
         hipLaunchKernelGGL((attn_merge_kernel<S, In>), dim3(blocks_for(rows * D, 256)), dim3(256), 0, s,
-                           (const S *)a.po, (const S *)a.pm, (const S *)a.pl, rows, D, (In *)out, row_max, row_sum);
+                           (const S *)a.po, (const S *)a.pm, (const S *)a.pl, rows, D, (In *)out, (S *)row_max,
+                           (S *)row_sum);
     }
     EF_HIP(hipGetLastError());
     return EF_OK;
@@ -1342,7 +1358,7 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
 
 template <int P>
 ef_status dispatch_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, void *out,
-                       float *m, float *l, uint64_t *vis, char *scratch, hipStream_t s) {
+                       void *m, void *l, uint64_t *vis, char *scratch, hipStream_t s) {
     switch (d.features) {
     case 32: return run_fwd<P, 32>(d, q, k, v, out, m, l, vis, scratch, s);
     case 64: return run_fwd<P, 64>(d, q, k, v, out, m, l, vis, scratch, s);
@@ -1357,7 +1373,7 @@ ef_status dispatch_fwd(const ef_attention_desc &d, const void *q, const void *k,
 
 template <int P, int D>
 ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, const void *out,
-                  const void *dout, const float *row_max, const float *row_sum, void *dq, void *dk, void *dv,
+                  const void *dout, const void *row_max, const void *row_sum, void *dq, void *dk, void *dv,
                   char *scratch, int stages, hipStream_t s) {
     typedef typename Input<P>::T In;
     typedef typename Elem<P>::T E;
@@ -1369,7 +1385,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
     a.qr = scratch + L.qr; a.qt = scratch + L.qt; a.dor = scratch + L.dor; a.dot = scratch + L.dot;
     a.kr = scratch + L.bkr; a.kt = scratch + L.bkt; a.vr = scratch + L.bvr;
     a.row_max = row_max; a.row_sum = row_sum;
-    a.delta = reinterpret_cast<float *>(scratch + L.delta);
+    a.delta = scratch + L.delta;
     a.dq = dq; a.dk = dk; a.dv = dv;
     a.mask = make_mask(d);
     a.tq = tq; a.tk = tk; a.tqp = tqp; a.tkp = tkp; a.batches = B;
@@ -1391,7 +1407,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         pa.nzero = 2;
         pa.out = out;
         pa.dout = dout;
-        pa.delta = (float *)a.delta;
+        pa.delta = scratch + L.delta;
         pa.tq = tq;
         const int64_t xb = (tqp > tkp ? tqp : tkp) / 64;
         hipLaunchKernelGGL((prep_kernel<P, D>), dim3((unsigned)xb, (unsigned)B, 5), dim3(256), 0, s, pa);
@@ -1433,7 +1449,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
 
 template <int P>
 ef_status dispatch_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, const void *out,
-                       const void *dout, const float *m, const float *l, void *dq, void *dk, void *dv, char *scratch,
+                       const void *dout, const void *m, const void *l, void *dq, void *dk, void *dv, char *scratch,
                        int stages, hipStream_t s) {
     switch (d.features) {
     case 32: return run_bwd<P, 32>(d, q, k, v, out, dout, m, l, dq, dk, dv, scratch, stages, s);
@@ -1449,7 +1465,7 @@ ef_status dispatch_bwd(const ef_attention_desc &d, const void *q, const void *k,
 size_t attention_scratch_bytes(const ef_attention_desc &d) { return layout(d).total; }
 
 ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, void *out,
-                        float *row_max, float *row_sum, uint64_t *tiles_visited, void *scratch, hipStream_t s) {
+                        void *row_max, void *row_sum, uint64_t *tiles_visited, void *scratch, hipStream_t s) {
     if (d.batches == 0 || d.queries == 0) return EF_OK;
     if (fast_fwd_shape(d) && d.precision <= EF_PRECISION_BF16) {
         if (!scratch) {
@@ -1471,12 +1487,14 @@ ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k
     const size_t threads = d.batches * d.queries * (size_t)chunks;
     if (d.precision == EF_PRECISION_FP64)
         hipLaunchKernelGGL(attn_fwd_generic<double>, dim3(blocks_for(threads, 256)), dim3(256), 0, s,
-                           (const double *)q, (const double *)k, (const double *)v, (double *)out, row_max, row_sum,
+                           (const double *)q, (const double *)k, (const double *)v, (double *)out, (double *)row_max,
+                           (double *)row_sum,
                            mask, (int64_t)d.batches, (int64_t)d.queries, (int64_t)d.keys, (int64_t)d.features,
                            (int64_t)d.values, d.scale);
     else
         hipLaunchKernelGGL(attn_fwd_generic<float>, dim3(blocks_for(threads, 256)), dim3(256), 0, s,
-                           (const float *)q, (const float *)k, (const float *)v, (float *)out, row_max, row_sum,
+                           (const float *)q, (const float *)k, (const float *)v, (float *)out, (float *)row_max,
+                           (float *)row_sum,
                            mask, (int64_t)d.batches, (int64_t)d.queries, (int64_t)d.keys, (int64_t)d.features,
                            (int64_t)d.values, d.scale);
     EF_HIP(hipGetLastError());
@@ -1484,7 +1502,7 @@ ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k
 }
 
 ef_status attention_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, const void *out,
-                        const void *dout, const float *row_max, const float *row_sum, void *dq, void *dk, void *dv,
+                        const void *dout, const void *row_max, const void *row_sum, void *dq, void *dk, void *dv,
                         void *scratch, int stages, hipStream_t s) {
     if (d.batches == 0) return EF_OK;
     if (!fast_bwd_shape(d)) {

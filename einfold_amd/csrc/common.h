@@ -46,7 +46,13 @@ struct DeviceBuffer {
         bytes = n;
         if (n == 0) return EF_OK;
         hipError_t e = hipMalloc(&ptr, n);
-        if (e != hipSuccess) { ptr = nullptr; return hip_status(e, "hipMalloc"); }
+        if (e != hipSuccess) {
+            // Reset the thread's last error, so that a later launch's
+            // hipGetLastError() does not report this refusal again.
+            (void)hipGetLastError();
+            ptr = nullptr;
+            return hip_status(e, "hipMalloc");
+        }
         return EF_OK;
     }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
@@ -74,12 +80,13 @@ size_t rank_scratch_bytes(size_t n);
 ef_status sort_u64(const uint64_t *words, const uint32_t *rows_in, size_t n, uint64_t *w_out, uint32_t *r_out,
                    void *scratch, hipStream_t s);
 size_t attention_scratch_bytes(const ef_attention_desc &d);
+// Row statistics are float per query, double for EF_PRECISION_FP64.
 ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
-                        void *out, float *row_max, float *row_sum, uint64_t *tiles_visited,
+                        void *out, void *row_max, void *row_sum, uint64_t *tiles_visited,
                         void *scratch, hipStream_t s);
 ef_status attention_bwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
-                        const void *out, const void *dout, const float *row_max,
-                        const float *row_sum, void *dq, void *dk, void *dv, void *scratch,
+                        const void *out, const void *dout, const void *row_max,
+                        const void *row_sum, void *dq, void *dk, void *dv, void *scratch,
                         int stages, hipStream_t s);   // stages: 1 prepare, 2 dK/dV, 4 dQ
 
 struct GroupedPlan;

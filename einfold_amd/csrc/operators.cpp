@@ -835,7 +835,8 @@ ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Re
     size_t kz = 0;
     EF_TRY(rank_columns(c, zptr, na + nb, z_rank, z_first, kz, rscratch));
 
-    // The outputs' keys, each operand's with the domain's.
+    // Output key columns to rank: an operand's own values, followed by the
+    // domain's values for the same outputs when a domain is given.
     std::deque<DevColumn> xcols, ycols;
     std::vector<DevColumn *> xptr, yptr;
     for (int side = 0; side < 2; side++) {
@@ -870,8 +871,8 @@ ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Re
     const int64_t nx = (int64_t)std::max<size_t>(kx, 1), nz = (int64_t)std::max<size_t>(kz, 1),
                   ny = (int64_t)std::max<size_t>(ky, 1);
 
-    // a's joining rows in (x, z) order, b's in (z, y) order, duplicates by
-    // their factors.
+    // Sort the rows that join: a by (x rank, z rank), b by (z rank, y rank);
+    // rows whose ranks tie are ordered by their factor bits.
     uint32_t *flags32, *excl32, *rows_a, *rows_b;
     uint8_t *scan_bytes;
     const int64_t nmax = std::max<int64_t>(std::max(na, nb), 1);
@@ -1390,7 +1391,7 @@ ef_status pipeline_impl(const ef_pipeline_desc *pd, const FoldModel &prod, const
         d.allowed = dspans;
         const uint32_t prec = c.options ? c.options->precision : EF_PRECISION_FP64;
         d.precision = (any32 && prec != EF_PRECISION_FP64) ? prec : EF_PRECISION_FP64;
-        float *rm, *rs;
+        double *rm, *rs;   // row statistics: float, or double for FP64
         EF_TRY(c.alloc(rm, nx));
         EF_TRY(c.alloc(rs, nx));
         uint8_t *scratch;
@@ -1656,7 +1657,7 @@ std::vector<uint32_t> chain_kept(const FoldModel &m, const std::vector<uint32_t>
         if (m.output_of(d) >= 0 || m.operands[rest].column_of(d) >= 0) kept.push_back(d);
     return kept;
 }
-// `op` with its dimensions numbered by their place in `local`.
+// A copy of `op` whose keys name each dimension by its index in `local`.
 OperandModel chain_renumber(const OperandModel &op, const std::vector<uint32_t> &local) {
     OperandModel r;
     r.factor = op.factor;

@@ -118,8 +118,9 @@ def attention_fwd(q: torch.Tensor, k: torch.Tensor, v: torch.Tensor, scale: floa
     if allowed is not None:
         _need(allowed, torch.int64, (tq, 2), "allowed")
     out = torch.empty((b, tq, dv), dtype=dtype, device=q.device)
-    m = torch.empty((b, tq), dtype=torch.float32, device=q.device)
-    l = torch.empty((b, tq), dtype=torch.float32, device=q.device)
+    # Row statistics in the precision's accumulation type (double for FP64).
+    m = torch.empty((b, tq), dtype=dtype, device=q.device)
+    l = torch.empty((b, tq), dtype=dtype, device=q.device)
     visited = torch.zeros(1, dtype=torch.int64, device=q.device) if count_tiles else None
     desc = _desc(b, tq, tk, d, dv, scale, precision, causal, offset, window, allowed)
     if scratch is None or scratch.numel() < lib().ef_attention_scratch_bytes(ctypes.byref(desc)):
@@ -144,8 +145,8 @@ def attention_bwd(q, k, v, out, dout, row_max, row_sum, scale: float, causal: bo
     for t, shp, nm in ((q, (b, tq, d), "q"), (k, (b, tk, d), "k"), (v, (b, tk, dv_), "v"),
                        (out, (b, tq, dv_), "out"), (dout, (b, tq, dv_), "dout")):
         _need(t, dtype, shp, nm)
-    _need(row_max, torch.float32, (b, tq), "row_max")
-    _need(row_sum, torch.float32, (b, tq), "row_sum")
+    _need(row_max, dtype, (b, tq), "row_max")
+    _need(row_sum, dtype, (b, tq), "row_sum")
     desc = _desc(b, tq, tk, d, dv_, scale, precision, causal, offset, window, allowed)
     if scratch is None or scratch.numel() < lib().ef_attention_scratch_bytes(ctypes.byref(desc)):
         scratch = attention_scratch(desc, q.device)

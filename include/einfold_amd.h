@@ -302,9 +302,11 @@ ef_status ef_rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t 
  * (key <= query + causal_offset), else also key > query - window.
  * Scores are scaled by `scale` (multiplied).  Writes out
  * [batches][queries][values] and per-row statistics row_max and row_sum
- * (the softmax's shift and normalizer, for the backward pass); a query
- * with no allowed key gets NaN.  precision: EF_PRECISION_FP32, _BF16X3,
- * _BF16 (float32 tensors) or EF_PRECISION_FP64 (double tensors). */
+ * [batches][queries] (the softmax's shift and normalizer, for the
+ * backward pass): float32 for EF_PRECISION_FP32, _BF16X3 and _BF16,
+ * float64 for EF_PRECISION_FP64.  A query with no allowed key gets NaN.
+ * precision: EF_PRECISION_FP32, _BF16X3, _BF16 (float32 tensors) or
+ * EF_PRECISION_FP64 (double tensors). */
 typedef struct ef_attention_desc {
     size_t batches, queries, keys, features, values;
     double scale;
@@ -323,7 +325,7 @@ typedef struct ef_attention_desc {
 size_t ef_attention_scratch_bytes(const ef_attention_desc *desc);
 
 ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const void *k,
-                           const void *v, void *out, float *row_max, float *row_sum,
+                           const void *v, void *out, void *row_max, void *row_sum,
                            uint64_t *tiles_visited, void *scratch, void *stream);
 
 /* The backward pass of ef_attention_fwd, recomputing score tiles from the
@@ -331,7 +333,7 @@ ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const v
  * (the upstream gradient), writes dq, dk, dv.  32 or 64 features. */
 ef_status ef_attention_bwd(const ef_attention_desc *desc, const void *q, const void *k,
                            const void *v, const void *out, const void *dout,
-                           const float *row_max, const float *row_sum,
+                           const void *row_max, const void *row_sum,
                            void *dq, void *dk, void *dv, void *scratch, void *stream);
 
 /* ef_attention_bwd in its three stages, for callers that want only some
@@ -342,10 +344,10 @@ ef_status ef_attention_bwd_prepare(const ef_attention_desc *desc, const void *q,
                                    const void *v, const void *out, const void *dout,
                                    void *scratch, void *stream);
 ef_status ef_attention_bwd_kv(const ef_attention_desc *desc, const void *k, const void *v,
-                              const float *row_max, const float *row_sum, void *scratch,
+                              const void *row_max, const void *row_sum, void *scratch,
                               void *dk, void *dv, void *stream);
 ef_status ef_attention_bwd_q(const ef_attention_desc *desc, const void *q, const void *dout,
-                             const float *row_max, const float *row_sum, void *scratch,
+                             const void *row_max, const void *row_sum, void *scratch,
                              void *dq, void *stream);
 
 /* Many uneven FP32 products in one persistent launch (batched dense folds

@@ -1,14 +1,13 @@
 """The backward kernels through torch.autograd, checked three ways:
 
 * FP64 central differences (eps = 1e-3) of SUM(out * g) on 256 random
-  entries of each of q, k and v, at 4 heads x T = 2048 x d = 64;
-  the saved row statistics are float32, so the FP64 form's gradients are
-  within about 1e-7 relative, not to the last bit;
+  entries of each of q, k and v, at 4 heads x T = 2048 x d = 64 (the
+  differences' own O(eps^2) truncation sets the 1e-5 bound);
 * torch float64 autograd of the same attention on the CPU;
-* the left-joined gradient folds that ddx writes for dV
-  (docs/spikes/s31-ddx-gradient-steps): dV(h, tk, d) = SUM over tq of
-  P(h, tq, tk) * dO(h, tq, d), left-joined from the v table with 0.0 for
-  keys no query reaches, run by the dense fold operator (ef_fold_dense).
+* the left-joined gradient folds that ddx writes for dQ, dK and dV
+  (docs/spikes/s31-ddx-gradient-steps), each left-joined from the
+  differentiated table with 0.0 for positions nothing reaches, run by the
+  dense fold operator (ef_fold_dense).
 """
 import numpy as np
 import pyarrow as pa
@@ -55,7 +54,7 @@ def test_central_differences_fp64(cuda):
         assert err < 1e-5, (name, err)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("fp32", 1e-5), ("bf16x3", 3e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16x3", 3e-5)])
 def test_autograd_against_torch_float64(cuda, precision, tol):
     h, t, d, scale = 2, 384, 64, 0.125
     q, k, v, g = _tables(h, t, d, 6)
@@ -74,34 +73,55 @@ def test_autograd_against_torch_float64(cuda, precision, tol):
         assert err < tol, err
 
 
-def test_dv_equals_the_left_joined_gradient_fold(cuda):
-    """dV from the backward kernel (FP64) against ddx's gradient SQL for dV
-    run as a dense fold from the v table's domain with missing 0.0."""
+@pytest.mark.parametrize("wrt", ["q", "k", "v"])
+def test_gradients_equal_the_left_joined_gradient_folds(cuda, wrt):
+    """dQ, dK and dV from the backward kernels (FP64) against ddx's gradient
+    SQL (docs/spikes/s31-ddx-gradient-steps/README.md:57-67), each run as a
+    dense fold left-joined from the differentiated table with missing 0.0
+    (crates/einfold-datafusion/tests/detect_vs_host.rs:1306-1367):
+
+      dQ(h, tq, d) = SUM over tk of dS(h, tq, tk) * scale * K(h, tk, d)
+      dK(h, tk, d) = SUM over tq of dS(h, tq, tk) * scale * Q(h, tq, d)
+      dV(h, tk, d) = SUM over tq of P(h, tq, tk) * dO(h, tq, d)
+
+    with P the causal softmax and dS = P * (dP - rowsum(dO * O)), dP = dO V^T,
+    given as tables of the allowed (tq, tk) pairs only: the masked triangle
+    is holes, which the domain's default turns into 0.0."""
     h, t, d, scale = 2, 256, 32, 0.125
     q, k, v, g = _tables(h, t, d, 7)
     xs = [torch.from_numpy(np.ascontiguousarray(x)).to(cuda).requires_grad_(True) for x in (q, k, v)]
     _loss(*xs, torch.from_numpy(g).to(cuda), scale, "fp64").backward()
-    dv = xs[2].grad.cpu().numpy()
+    grad = xs["qkv".index(wrt)].grad.cpu().numpy()
     s = scale * np.einsum("htd,hsd->hts", q, k)
-    s = np.where(np.tril(np.ones((t, t), bool)), s, -np.inf)
+    allowed = np.tril(np.ones((t, t), bool))
+    s = np.where(allowed, s, -np.inf)
     p = np.exp(s - s.max(-1, keepdims=True))
     p /= p.sum(-1, keepdims=True)
-    hh, tq, tk = np.nonzero(p > 0)
-    ptab = pa.table({"h": hh.astype(np.int64), "tq": tq.astype(np.int64), "tk": tk.astype(np.int64),
-                     "w": p[hh, tq, tk]})
+    o = np.einsum("hts,hsd->htd", p, v)
+    dp = np.einsum("htd,hsd->hts", g, v)
+    ds = p * (dp - (g * o).sum(-1, keepdims=True))
+    hh, tq, tk = np.nonzero(np.broadcast_to(allowed, (h, t, t)))
+    w = (p if wrt == "v" else ds * scale)[hh, tq, tk]
+    ptab = pa.table({"h": hh.astype(np.int64), "tq": tq.astype(np.int64), "tk": tk.astype(np.int64), "w": w})
     gh, gt, gd = np.meshgrid(np.arange(h), np.arange(t), np.arange(d), indexing="ij")
-    gtab = pa.table({"h": gh.reshape(-1), "tq": gt.reshape(-1), "d": gd.reshape(-1), "v": g.reshape(-1)})
-    vtab = pa.table({"h": gh.reshape(-1), "tk": gt.reshape(-1), "d": gd.reshape(-1)})
+    cols = lambda name, x: {"h": gh.reshape(-1), name: gt.reshape(-1), "d": gd.reshape(-1), "v": x.reshape(-1)}
+    other = {"q": pa.table(cols("tk", k)), "k": pa.table(cols("tq", q)), "v": pa.table(cols("tq", g))}[wrt]
+    dom = pa.table({"h": gh.reshape(-1), "t": gt.reshape(-1), "d": gd.reshape(-1)})
     EQ = ir.KeyEquality.EQUAL
-    # dims 0 = h, 1 = tq, 2 = tk, 3 = d; GROUP BY h, tk, d from the domain v.
-    fold = ir.Fold([EQ] * 4, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1), ir.Key(2, 2)), 3),
-                              ir.Operand((ir.Key(0, 0), ir.Key(1, 1), ir.Key(3, 2)), 3)],
-                   [ir.Output(0, ir.DomainKey(0)), ir.Output(2, ir.DomainKey(1)), ir.Output(3, ir.DomainKey(2))],
+    # dims 0 = h, 1 = tq, 2 = tk, 3 = d
+    a_op = ir.Operand((ir.Key(0, 0), ir.Key(1, 1), ir.Key(2, 2)), 3)
+    if wrt == "q":   # K's rows are keys: joined on tk, grouped by (h, tq, d)
+        b_op = ir.Operand((ir.Key(0, 0), ir.Key(2, 1), ir.Key(3, 2)), 3)
+        outs = [0, 1, 3]
+    else:            # Q's or dO's rows are queries: joined on tq, grouped by (h, tk, d)
+        b_op = ir.Operand((ir.Key(0, 0), ir.Key(1, 1), ir.Key(3, 2)), 3)
+        outs = [0, 2, 3]
+    fold = ir.Fold([EQ] * 4, [a_op, b_op], [ir.Output(x, ir.DomainKey(i)) for i, x in enumerate(outs)],
                    domain=ir.Domain([0, 1, 2], 0.0))
-    out = kernels.fold_dense(fold, ptab, gtab, vtab)
-    assert out is not None
+    out = kernels.fold_dense(fold, ptab, other, dom)
+    assert out is not None, "the dense algorithm takes the masked operand under the 0.0 default"
     tab = pa.Table.from_batches(out)
-    got = tab.column(3).to_numpy().reshape(h, t, d)
     assert np.array_equal(tab.column(1).to_numpy().reshape(h, t, d)[0, :, 0], np.arange(t))
-    err = np.max(np.abs(got - dv)) / np.max(np.abs(dv))
-    assert err < 1e-6, err
+    got = tab.column(3).to_numpy().reshape(h, t, d)
+    err = np.max(np.abs(got - grad)) / np.max(np.abs(grad))
+    assert err < 1e-11, (wrt, err)

@@ -44,7 +44,10 @@ def test_matmul(cuda, shape):
     assert be.calls == [("matmul", shape)]
 
 
-@pytest.mark.parametrize("code,c", [(0, 1.0), (1, 0.125), (2, 8.0)])
+# Scale::Over with c = 3 (no exact reciprocal): the device multiplies by the
+# rounded 1/c, one more rounding per score than the division, inside the
+# test's tolerance.
+@pytest.mark.parametrize("code,c", [(0, 1.0), (1, 0.125), (2, 8.0), (2, 3.0), (1, 0.3)])
 @pytest.mark.parametrize("shape,mask", [((2, 200, 64, 200, 64), "causal"), ((1, 100, 16, 150, 8), "window"),
                                         ((3, 70, 8, 90, 5), "none")])
 def test_attention(cuda, shape, mask, code, c):

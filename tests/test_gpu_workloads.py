@@ -1,0 +1,185 @@
+"""The configurations' own workloads and factor types through the operator
+entry points, against the oracle on the same Arrow inputs:
+
+* Float32 factors (configs 2 and 5 give FLOAT32 values) through
+  ef_fold_dense and ef_fold.  The reference declines Float32 factors
+  (crates/einfold-kernels/src/discriminate.rs:407-412, dense.rs via
+  schema.rs:77-85), so its answer for them is SQL's; with the FP64
+  precision the device widens each factor to Float64 and multiplies and sums
+  in Float64 (a product of two widened Float32 values is exact), so the
+  result is the oracle's on the widened tables bit for bit;
+* config 5's regridding fold: w(src INT32, dst INT32, w FLOAT32) with 16
+  random src per dst applied to x(src, t, v FLOAT32) over several time
+  steps, through ef_fold's sorting path;
+* config 2's sparse product at 10^5 rows per side: UInt64 hashed keys, the
+  join key Zipf(1.1)-distributed, FLOAT32 factors; bit-identical to the
+  oracle and across record batches of 8192 and 65536 rows
+  (crates/einfold-kernels/tests/kernels_vs_sql.rs:895-1002);
+* device memory running short: EF_OUT_OF_MEMORY (the caller's host plan),
+  never a fault, and the device keeps working afterwards.
+"""
+import ctypes
+
+import numpy as np
+import pyarrow as pa
+import pyarrow.compute as pc
+import pytest
+
+from einfold_amd import _lib, backend, data, ir, kernels
+from oracle import reference as R
+
+from test_gpu_ops import assert_same
+
+pytestmark = pytest.mark.gpu
+
+EQ = ir.KeyEquality.EQUAL
+
+
+def _matmul_fold(domain=None):
+    return ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                              ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)],
+                   [ir.Output(0), ir.Output(2)], domain=domain)
+
+
+def _widened(t: pa.Table) -> pa.Table:
+    """The table with every Float32 column cast to Float64 (exact)."""
+    cols = [pc.cast(c, pa.float64()) if c.type == pa.float32() else c for c in t.columns]
+    return pa.table(cols, names=t.column_names)
+
+
+def _f32_grid(n0, n1, names, seed, salt, null_every=0):
+    v = (data.unit(seed, salt, n0 * n1) * 2 - 1).astype(np.float32)
+    mask = (np.arange(n0 * n1) % null_every == 3) if null_every else None
+    return pa.table({names[0]: np.repeat(np.arange(n0, dtype=np.int64), n1),
+                     names[1]: np.tile(np.arange(n1, dtype=np.int64), n0),
+                     names[2]: pa.array(v, pa.float32(), mask=mask)})
+
+
+F32 = kernels.Options(accept_float32=True)
+
+
+def test_float32_factors_fold_dense_bits_of_the_widened_oracle(cuda):
+    print("float32 fold_dense 200x150x120", flush=True)
+    a = _f32_grid(200, 150, "ijv", 3, 1, null_every=17)
+    b = _f32_grid(150, 120, "jkv", 3, 2)
+    f = _matmul_fold()
+    got = kernels.fold_dense(f, a.to_batches(max_chunksize=4000), b, options=F32)
+    wa, wb = _widened(a), _widened(b)
+    want = R.fold_dense(f, wa.to_batches(), wb.to_batches(), None, R.schema(f, [wa.schema, wb.schema]))
+    assert_same(got, want)
+    # Without accept_float32 the schema is the reference's, which takes
+    # Float64 factors only (schema.rs:77-85): an error, not a decline.
+    with pytest.raises(ValueError, match="Float64"):
+        kernels.fold_dense(f, a, b)
+
+
+def test_float32_factors_fold_sorting_path_bits_of_the_widened_oracle(cuda):
+    print("float32 fold sorting path", flush=True)
+    rng = np.random.default_rng(31)
+    n = 3000
+    a = pa.table({"i": pa.array(rng.integers(0, 400, n)), "j": pa.array(rng.integers(0, 900, n)),
+                  "v": pa.array(rng.standard_normal(n).astype(np.float32), mask=rng.random(n) < 0.05)})
+    b = pa.table({"j": pa.array(rng.integers(0, 900, n)), "k": pa.array(rng.integers(0, 300, n)),
+                  "v": pa.array(rng.standard_normal(n).astype(np.float32))})
+    f = _matmul_fold()
+    got = kernels.fold(f, a, b, options=F32)
+    wa, wb = _widened(a), _widened(b)
+    algo, want = R.fold(f, wa.to_batches(), wb.to_batches(), None, R.schema(f, [wa.schema, wb.schema]))
+    assert algo == got.algorithm == "discriminate"
+    assert_same(got.batches, want)
+
+
+def _regrid_tables(n_dst, per, n_src, steps, seed):
+    dst = np.repeat(np.arange(n_dst, dtype=np.int32), per)
+    src = (data.next_u64(seed, 1, n_dst * per) % np.uint64(n_src)).astype(np.int32)
+    w = pa.table({"dst": dst, "src": src, "w": data.unit(seed, 2, n_dst * per).astype(np.float32)})
+    x = pa.table({"src": np.repeat(np.arange(n_src, dtype=np.int32), steps),
+                  "t": np.tile(np.arange(steps, dtype=np.int32), n_src),
+                  "v": data.normal(seed, 3, n_src * steps).astype(np.float32)})
+    return w, x
+
+
+def test_regridding_fold_against_the_oracle(cuda):
+    """out(dst, t) = SUM(w.w * x.v) FROM w JOIN x ON w.src = x.src GROUP BY
+    dst, t: 4000 dst cells, 16 random src each (repeats allowed) over 3000
+    src cells, 24 time steps; Int32 keys, Float32 weights and values.  The
+    dense algorithm declines it (4000 x 3000 cells against 64k weight
+    rows), so it runs the sorting path, as at full size."""
+    print("regridding 4000 dst x 16 src x 24 steps", flush=True)
+    w, x = _regrid_tables(4000, 16, 3000, 24, 7)
+    f = _matmul_fold()
+    got = kernels.fold(f, w.to_batches(max_chunksize=10000), x, options=F32)
+    ww, wx = _widened(w), _widened(x)
+    algo, want = R.fold(f, ww.to_batches(), wx.to_batches(), None, R.schema(f, [ww.schema, wx.schema]))
+    assert algo == got.algorithm == "discriminate"
+    assert_same(got.batches, want)
+    t = pa.Table.from_batches(got.batches)
+    assert t.num_rows == 4000 * 24
+    # Against the closed form: out[dst] = sum over its 16 (src, w) of w * x[src].
+    W = np.zeros((4000, 3000))
+    np.add.at(W, (w.column(0).to_numpy(), w.column(1).to_numpy()), w.column(2).to_numpy().astype(np.float64))
+    X = x.column(2).to_numpy().astype(np.float64).reshape(3000, 24)
+    dense = (W @ X).reshape(-1)
+    keys = t.column(0).to_numpy().astype(np.int64) * 24 + t.column(1).to_numpy()
+    mag = (np.abs(W) @ np.abs(X)).reshape(-1)[keys]
+    assert np.all(np.abs(t.column(2).to_numpy() - dense[keys]) <= 16 * np.finfo(float).eps * mag)
+
+
+def _config2_tables(rows, seed):
+    """Config 2's generator at `rows` per side: j = hash of Zipf(1.1) ids
+    over a tenth as many ids as rows; i, k = hashes of 512 ids each (so the
+    groups stay few while the hot j bucket joins thousands of rows each
+    side); v ~ N(0, 1) as Float32."""
+    ids = max(rows // 10, 16)
+    a = pa.table({"i": pa.array(data.hash64(data.zipf_ids(seed, 1, rows, 512, 0.0), 0x11), pa.uint64()),
+                  "j": pa.array(data.hash64(data.zipf_ids(seed, 2, rows, ids, 1.1), 0x6A), pa.uint64()),
+                  "v": pa.array(data.normal(seed, 3, rows).astype(np.float32))})
+    b = pa.table({"j": pa.array(data.hash64(data.zipf_ids(seed, 4, rows, ids, 1.1), 0x6A), pa.uint64()),
+                  "k": pa.array(data.hash64(data.zipf_ids(seed, 5, rows, 512, 0.0), 0x22), pa.uint64()),
+                  "v": pa.array(data.normal(seed, 6, rows).astype(np.float32))})
+    return a, b
+
+
+def test_config2_sparse_product_at_1e5_rows_deterministic_across_batches(cuda):
+    print("config 2 sparse product 100k x 100k", flush=True)
+    rows = 100_000
+    a, b = _config2_tables(rows, 5)
+    f = _matmul_fold()
+    got8 = kernels.fold(f, a.to_batches(max_chunksize=8192), b.to_batches(max_chunksize=8192), options=F32)
+    got64 = kernels.fold(f, a.to_batches(max_chunksize=65536), b.to_batches(max_chunksize=65536), options=F32)
+    assert got8.algorithm == got64.algorithm == "discriminate"
+    assert_same(got8.batches, got64.batches)
+    wa, wb = _widened(a), _widened(b)
+    algo, want = R.fold(f, wa.to_batches(), wb.to_batches(), None, R.schema(f, [wa.schema, wb.schema]))
+    assert algo == "discriminate"
+    assert_same(got8.batches, want)
+    # The hot join key really is hot: thousands of rows on each side.
+    jj = wa.column(1).to_numpy()
+    _, counts = np.unique(jj, return_counts=True)
+    assert counts.max() > 5000
+
+
+def test_device_memory_short_is_out_of_memory_not_a_fault(cuda):
+    """Work larger than the device's memory: the DenseBackend product and
+    the attention forward's own scratch each ask hipMalloc for terabytes,
+    which fails before any copy or launch; the status is EF_OUT_OF_MEMORY
+    (the caller's host plan) and the device still runs the next call."""
+    L = _lib.lib()
+    tiny = np.zeros(16, np.float64)
+    P = lambda x: ctypes.c_void_p(x.ctypes.data)
+    shape = _lib.CProductShape(1, 1 << 20, 1 << 20, 1)   # a is 8 TiB
+    assert L.ef_dense_run_product(P(tiny), P(tiny), shape, P(tiny), None) == _lib.OUT_OF_MEMORY
+    desc = _lib.CAttentionDesc()
+    desc.batches, desc.queries, desc.keys, desc.features, desc.values = 1 << 14, 1 << 20, 1 << 20, 64, 64
+    desc.scale, desc.causal, desc.precision = 0.125, 1, _lib.PRECISION["fp32"]
+    assert L.ef_attention_scratch_bytes(ctypes.byref(desc)) > (1 << 42)
+    assert L.ef_attention_fwd(ctypes.byref(desc), None, None, None, None, None, None, None, None, None) \
+        == _lib.OUT_OF_MEMORY
+    assert b"hipMalloc" in L.ef_last_error()
+    # The backend still takes a small product.
+    be = backend.Mi355xBackend()
+    assert be("matmul", [tiny[:4], tiny[:4]], (1, 2, 2, 2)) is not None
+    # And the device works on.
+    a, b = data.grid_matmul(32)
+    out = kernels.fold_dense(_matmul_fold(), a, b)
+    assert out is not None and sum(x.num_rows for x in out) == 32 * 32
