@@ -27,7 +27,7 @@ EXPORTS = [
     "ef_dense_accumulate_f64", "ef_dense_product_f32",
     "ef_rank_scratch_bytes", "ef_rank_u64",
     "ef_attention_scratch_bytes", "ef_attention_fwd", "ef_attention_bwd",
-    "ef_attention_bwd_prepare", "ef_attention_bwd_kv", "ef_attention_bwd_q",
+    "ef_attention_bwd_prepare", "ef_attention_bwd_kv", "ef_attention_bwd_q", "ef_attention_fwd_noalloc",
     "ef_grouped_plan_create", "ef_grouped_plan_run", "ef_grouped_plan_tile", "ef_grouped_plan_destroy",
 ]
 
@@ -165,6 +165,8 @@ def lib():
         L.ef_attention_scratch_bytes.restype = sz
         L.ef_attention_fwd.argtypes = [D, P, P, P, P, P, P, P, P, P]
         L.ef_attention_fwd.restype = st
+        L.ef_attention_fwd_noalloc.argtypes = [D, P, P, P, P, P, P, P, P]
+        L.ef_attention_fwd_noalloc.restype = st
         L.ef_attention_bwd.argtypes = [D, P, P, P, P, P, P, P, P, P, P, P, P]
         L.ef_attention_bwd.restype = st
         L.ef_attention_bwd_prepare.argtypes = [D, P, P, P, P, P, P, P]

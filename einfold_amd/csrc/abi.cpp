@@ -57,6 +57,9 @@ ef_status ef_init(int device) {
     clear_error();
     EF_HIP(hipSetDevice(device));
     EF_HIP(hipFree(nullptr));
+    // This device's address of the no-allocation entry point's array.
+    void *fixed = nullptr;
+    EF_TRY(noalloc_scratch(&fixed));
     return EF_OK;
 }
 

@@ -1268,14 +1268,15 @@ size_t elem_bytes(uint32_t precision) {
 }
 
 // Scratch layout, each region 256-byte aligned: the work counters, then
-// the forward's prepared K and V^T, or (from the same offset) the
-// backward's prepared tiles and each query's delta; then the two parts'
-// partial results (forward sums, max and sum; backward dK, dV, dQ).
+// the forward's prepared K and V^T and its two parts' partial results
+// (sums, max and sum), or (from the same offset) the backward's prepared
+// tiles, each query's delta and its parts' dK, dV and dQ.  The forward
+// needs only its own regions (fwd_total bytes).
 struct Layout {
     size_t counters, kr, vt;
     size_t qr, qt, dor, dot, bkr, bkt, bvr, delta;
     size_t po, pm, pl, pdk, pdv, pdq;
-    size_t total;
+    size_t fwd_total, total;
 };
 
 Layout layout(const ef_attention_desc &d) {
@@ -1283,11 +1284,16 @@ Layout layout(const ef_attention_desc &d) {
     const size_t e = elem_bytes(d.precision);
     const size_t tqp = (size_t)round_up((int64_t)d.queries, PAD), tkp = (size_t)round_up((int64_t)d.keys, PAD);
     const size_t qa = align256(d.batches * tqp * d.features * e), ka = align256(d.batches * tkp * d.features * e);
+    const size_t sb = d.precision == EF_PRECISION_FP64 ? 8 : 4;
+    const size_t rq = 2 * d.batches * d.queries, rk = 2 * d.batches * d.keys;
     size_t at = 0;
     L.counters = at; at += 256;
     L.kr = at; at += ka;
     L.vt = at; at += ka;
-    const size_t fwd_end = at;
+    L.po = at; at += align256(rq * d.values * sb);
+    L.pm = at; at += align256(rq * sb);
+    L.pl = at; at += align256(rq * sb);
+    L.fwd_total = at;
     at = 256;
     L.qr = at; at += qa;
     L.qt = at; at += qa;
@@ -1296,17 +1302,11 @@ Layout layout(const ef_attention_desc &d) {
     L.bkr = at; at += ka;
     L.bkt = at; at += ka;
     L.bvr = at; at += ka;
-    L.delta = at; at += align256(d.batches * d.queries * (d.precision == EF_PRECISION_FP64 ? 8 : 4));
-    at = at > fwd_end ? at : fwd_end;
-    const size_t sb = d.precision == EF_PRECISION_FP64 ? 8 : 4;
-    const size_t rq = 2 * d.batches * d.queries, rk = 2 * d.batches * d.keys;
-    L.po = at; at += align256(rq * d.values * sb);
-    L.pm = at; at += align256(rq * sb);
-    L.pl = at; at += align256(rq * sb);
+    L.delta = at; at += align256(d.batches * d.queries * sb);
     L.pdk = at; at += align256(rk * d.features * sb);
     L.pdv = at; at += align256(rk * d.values * sb);
     L.pdq = at; at += align256(rq * d.features * sb);
-    L.total = at;
+    L.total = at > L.fwd_total ? at : L.fwd_total;
     return L;
 }
 
@@ -1463,6 +1463,12 @@ ef_status dispatch_bwd(const ef_attention_desc &d, const void *q, const void *k,
 }  // namespace
 
 size_t attention_scratch_bytes(const ef_attention_desc &d) { return layout(d).total; }
+
+size_t attention_fwd_scratch_bytes(const ef_attention_desc &d) {
+    // The generic kernel (shapes the MFMA kernels don't take) uses none.
+    if (d.batches == 0 || d.queries == 0 || !(fast_fwd_shape(d) && d.precision <= EF_PRECISION_BF16)) return 0;
+    return layout(d).fwd_total;
+}
 
 ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v, void *out,
                         void *row_max, void *row_sum, uint64_t *tiles_visited, void *scratch, hipStream_t s) {

@@ -80,6 +80,10 @@ size_t rank_scratch_bytes(size_t n);
 ef_status sort_u64(const uint64_t *words, const uint32_t *rows_in, size_t n, uint64_t *w_out, uint32_t *r_out,
                    void *scratch, hipStream_t s);
 size_t attention_scratch_bytes(const ef_attention_desc &d);
+size_t attention_fwd_scratch_bytes(const ef_attention_desc &d);   // the forward pass's share
+// This device's address of the fixed scratch array of the no-allocation
+// entry point (resolved once per device).
+ef_status noalloc_scratch(void **ptr);
 // Row statistics are float per query, double for EF_PRECISION_FP64.
 ef_status attention_fwd(const ef_attention_desc &d, const void *q, const void *k, const void *v,
                         void *out, void *row_max, void *row_sum, uint64_t *tiles_visited,

@@ -328,6 +328,21 @@ ef_status ef_attention_fwd(const ef_attention_desc *desc, const void *q, const v
                            const void *v, void *out, void *row_max, void *row_sum,
                            uint64_t *tiles_visited, void *scratch, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  ef_attention_fwd with its scratch in a
+ * fixed device array of EF_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends) instead of a caller's buffer; the caller still passes the
+ * tensors and outputs.  It allocates nothing.  A description whose forward
+ * pass needs more scratch than the array holds is rejected (EF_INVALID,
+ * the needed bytes in ef_last_error()); at 8 heads x 4096 queries and keys
+ * x 64 features the FP32 and BF16 precisions need about 33 MB.  Every call
+ * on one GPU shares its array: calls that overlap on one GPU race. */
+#define EF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+ef_status ef_attention_fwd_noalloc(const ef_attention_desc *desc, const void *q, const void *k,
+                                   const void *v, void *out, void *row_max, void *row_sum,
+                                   uint64_t *tiles_visited, void *stream);
+
 /* The backward pass of ef_attention_fwd, recomputing score tiles from the
  * saved row_max/row_sum (never storing the T x T scores): given dout
  * (the upstream gradient), writes dq, dk, dv.  32 or 64 features. */
