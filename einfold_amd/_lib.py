@@ -81,7 +81,7 @@ class COptions(ctypes.Structure):
                 ("domain_rows", ctypes.c_uint32), ("precision", ctypes.c_uint32),
                 ("budget_ctx", ctypes.c_void_p), ("budget_grow", GROW),
                 ("budget_shrink", SHRINK), ("stream", ctypes.c_void_p),
-                ("accept_float32", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+                ("accept_float32", ctypes.c_uint32), ("fp64_mfma", ctypes.c_uint32)]
 
 
 class CArrowSchema(ctypes.Structure):

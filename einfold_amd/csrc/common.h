@@ -99,5 +99,10 @@ ef_status grouped_plan_create(const ef_gemm_problem *problems, size_t count, uin
 ef_status grouped_plan_run(GroupedPlan *plan, void *stream);
 int grouped_plan_tile(const GroupedPlan *plan);
 void grouped_plan_destroy(GroupedPlan *plan);
+// One set of products run once on `s` (no graph), with the autotuner's tile
+// shape for its (M, N, K, density) signature; returns after they finish.
+// density: per problem, or null for 1.
+ef_status grouped_run_once(const ef_gemm_problem *problems, size_t count, uint32_t precision, const double *density,
+                           hipStream_t s);
 
 }  // namespace ef

@@ -445,6 +445,7 @@ int tile_n(uint32_t precision, int config) {
 // prep + counter reset + GEMM on the plan's stream.
 struct GroupedPlan {
     std::vector<ef_gemm_problem> problems;
+    std::vector<double> density;      // per problem: operand rows over cells (1 for device tensors)
     uint32_t precision = EF_PRECISION_BF16X3;
     int config = 0;                   // tile shape: 0 = 128 x 128, 1 = 64 x 64, 2 = large (BF16 precisions)
     DeviceBuffer mem;
@@ -559,21 +560,33 @@ ef_status launch_all(const GroupedPlan &p, hipStream_t s) {
     }
 }
 
-// The autotuner's memory: the fastest tile shape per (precision, shape
-// signature), measured once per process.
+// The autotuner's memory: the fastest tile shape per (precision, set of
+// (M, N, K, key density)), measured once per process.
 std::mutex tune_mu;
 std::map<std::vector<int64_t>, int> tuned;
 
 std::vector<int64_t> signature(const GroupedPlan &p) {
-    // Shapes bucketed to powers of two, sorted: similar sets share a choice.
+    // Each problem's M, N and K bucketed to powers of two and its key
+    // density (the fold's operand rows over the cells they fill) to eighths,
+    // sorted: similar sets share a choice.
     std::vector<int64_t> sig = {(int64_t)p.precision};
     std::vector<int64_t> shapes;
     auto bucket = [](size_t v) { int64_t b = 0; while (((size_t)1 << b) < v) b++; return b; };
-    for (auto &q : p.problems) shapes.push_back(bucket(q.m) << 16 | bucket(q.n) << 8 | bucket(q.k));
+    for (size_t i = 0; i < p.problems.size(); i++) {
+        const ef_gemm_problem &q = p.problems[i];
+        const double d = i < p.density.size() ? p.density[i] : 1.0;
+        const int64_t eighths = d >= 1.0 ? 8 : (int64_t)(d * 8.0);
+        shapes.push_back(bucket(q.m) << 24 | bucket(q.n) << 16 | bucket(q.k) << 8 | eighths);
+    }
     std::sort(shapes.begin(), shapes.end());
     sig.insert(sig.end(), shapes.begin(), shapes.end());
     return sig;
 }
+
+// The tile shape for a plan: the caller's (64, 128 or, for the BF16
+// precisions, 256), else the autotuner's choice for its signature, timed
+// now if this process has not met the signature before.
+ef_status choose_config(GroupedPlan &p, int tile, hipStream_t s);
 
 ef_status time_config(GroupedPlan &p, hipStream_t s, float &ms) {
     EF_TRY(build(p, s));
@@ -593,6 +606,57 @@ ef_status time_config(GroupedPlan &p, hipStream_t s, float &ms) {
 
 }  // namespace
 
+namespace {
+
+ef_status choose_config(GroupedPlan &p, int tile, hipStream_t s) {
+    if (tile == 128 || tile == 64 || (tile == 256 && config_valid(p.precision, 2))) {
+        p.config = tile == 128 ? 0 : tile == 64 ? 1 : 2;
+        return EF_OK;
+    }
+    const auto sig = signature(p);
+    {
+        std::lock_guard<std::mutex> lock(tune_mu);
+        auto it = tuned.find(sig);
+        if (it != tuned.end()) {
+            p.config = it->second;
+            return EF_OK;
+        }
+    }
+    float best = 1e30f;
+    int best_cfg = 0;
+    for (int cfg = 0; cfg < 3; cfg++) {
+        if (!config_valid(p.precision, cfg)) continue;
+        p.config = cfg;
+        float ms = 0;
+        EF_TRY(time_config(p, s, ms));
+        if (ms < best) { best = ms; best_cfg = cfg; }
+    }
+    p.config = best_cfg;
+    std::lock_guard<std::mutex> lock(tune_mu);
+    tuned[sig] = best_cfg;
+    return EF_OK;
+}
+
+}  // namespace
+
+ef_status grouped_run_once(const ef_gemm_problem *problems, size_t count, uint32_t precision, const double *density,
+                           hipStream_t s) {
+    if (precision != EF_PRECISION_FP32 && precision != EF_PRECISION_BF16X3 && precision != EF_PRECISION_BF16) {
+        set_error("grouped products take FP32, BF16X3 or BF16");
+        return EF_INVALID;
+    }
+    GroupedPlan plan;
+    plan.problems.assign(problems, problems + count);
+    if (density) plan.density.assign(density, density + count);
+    plan.precision = precision;
+    EF_TRY(choose_config(plan, 0, s));
+    EF_TRY(build(plan, s));
+    EF_TRY(launch_all(plan, s));
+    // The plan's buffers are freed on return: wait for the launches.
+    EF_HIP(hipStreamSynchronize(s));
+    return EF_OK;
+}
+
 ef_status grouped_plan_create(const ef_gemm_problem *problems, size_t count, uint32_t precision, int tile,
                               void *stream, GroupedPlan **out) {
     *out = nullptr;
@@ -604,34 +668,7 @@ ef_status grouped_plan_create(const ef_gemm_problem *problems, size_t count, uin
     auto plan = new GroupedPlan();
     plan->problems.assign(problems, problems + count);
     plan->precision = precision;
-    ef_status st = EF_OK;
-    if (tile == 128 || tile == 64 || (tile == 256 && config_valid(precision, 2))) {
-        plan->config = tile == 128 ? 0 : tile == 64 ? 1 : 2;
-    } else {
-        const auto sig = signature(*plan);
-        int known = -1;
-        {
-            std::lock_guard<std::mutex> lock(tune_mu);
-            auto it = tuned.find(sig);
-            if (it != tuned.end()) known = it->second;
-        }
-        if (known >= 0) {
-            plan->config = known;
-        } else {
-            float best = 1e30f;
-            int best_cfg = 0;
-            for (int cfg = 0; cfg < 3 && st == EF_OK; cfg++) {
-                if (!config_valid(precision, cfg)) continue;
-                plan->config = cfg;
-                float ms = 0;
-                st = time_config(*plan, s, ms);
-                if (st == EF_OK && ms < best) { best = ms; best_cfg = cfg; }
-            }
-            plan->config = best_cfg;
-            std::lock_guard<std::mutex> lock(tune_mu);
-            tuned[sig] = best_cfg;
-        }
-    }
+    ef_status st = choose_config(*plan, tile, s);
     if (st == EF_OK) st = build(*plan, s);
     // One graph of the three launches, replayed by each run.
     if (st == EF_OK && plan->n_items > 0) {

@@ -545,11 +545,43 @@ ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation 
         }
     }
 
-    // The contraction (dense.rs:706-770), in the reference's order.
+    // The contraction (dense.rs:706-770): in the reference's order, or on
+    // MFMA where the caller asked for it and every factor is finite (the
+    // matrices hold 0 for absent and NULL cells, which then add nothing;
+    // with an infinite factor, 0 * inf would).
     double *out;
     EF_TRY(c.alloc(out, ncell));
-    if (ncell) EF_TRY(launch_dense_accumulate_f64(am, held, bm, nx, nz, ny, per_batch ? per_batch : 1, 1, out,
-                                                  c.stream));
+    const uint64_t pb = per_batch ? per_batch : 1;
+    const bool all_finite = finite && !pf.b_not_finite;
+    const uint32_t prec = c.options ? c.options->precision : EF_PRECISION_FP64;
+    const bool f32 = a.rows && b.rows && a.types[A.factor] == Type::Float32 && b.types[B.factor] == Type::Float32;
+    if (ncell == 0) {
+    } else if (f32 && prec != EF_PRECISION_FP64 && all_finite) {
+        // Float32 factors, FP32-accumulated products on the matrix cores:
+        // one grouped launch of the W batches' products (per_batch x nz)
+        // (nz x ny), its tile shape from the autotuner keyed on (M, N, K,
+        // key density); the placed Float64 values narrow back exactly.
+        float *a32, *b32, *c32;
+        EF_TRY(c.alloc(a32, na));
+        EF_TRY(c.alloc(b32, nb));
+        EF_TRY(c.alloc(c32, ncell));
+        EF_TRY(narrow_f32(am, (int64_t)na, a32, c.stream));
+        EF_TRY(narrow_f32(bm, (int64_t)nb, b32, c.stream));
+        const double density = std::min((double)a.rows / (double)std::max<uint64_t>(na, 1),
+                                        (double)b.rows / (double)std::max<uint64_t>(nb, 1));
+        std::vector<ef_gemm_problem> probs(nw);
+        std::vector<double> dens(nw, density);
+        for (uint64_t w = 0; w < nw; w++)
+            probs[w] = {a32 + w * pb * nz, b32 + w * nz * ny, c32 + w * pb * ny, (size_t)pb, (size_t)ny, (size_t)nz};
+        EF_TRY(grouped_run_once(probs.data(), probs.size(), prec, dens.data(), c.stream));
+        EF_TRY(to_f64(Type::Float32, c32, (int64_t)ncell, out, c.stream));
+    } else if (!f32 && c.options && c.options->fp64_mfma && all_finite && nw <= 65535) {
+        // Float64 factors on FP64 MFMA (opted into): another summation
+        // order, within 2 (n - 1) u sum|terms| of the exact one.
+        EF_TRY(launch_dense_product_f64(am, bm, out, nw, pb, ny, nz, c.stream));
+    } else {
+        EF_TRY(launch_dense_accumulate_f64(am, held, bm, nx, nz, ny, pb, 1, out, c.stream));
+    }
 
     if (domain) {
         // dense.rs:488-515: each domain row gets its cell's value where the

@@ -31,9 +31,9 @@ struct Dims {
 
 struct PlaceFlags {
     int duplicate;
-    int not_finite;
+    int not_finite;      // a's valued factors
     int nulls;
-    int pad;
+    int b_not_finite;    // b's valued factors
 };
 
 // How a domain's key columns locate a cell of the dense output.

@@ -102,8 +102,10 @@ __global__ __launch_bounds__(256) void place_a_kernel(int64_t n, Dims xs, Dims z
     }
 }
 
-// b's rows placed into [w][z][y] (dense.rs:372-386), holes left 0; only
-// a's factors need be finite (dense.rs:30-35).
+// b's rows placed into [w][z][y] (dense.rs:372-386), holes left 0.  Only
+// a's factors need be finite for the reference's order (dense.rs:30-35);
+// b's finiteness is flagged for the MFMA contractions, which multiply every
+// cell.
 __global__ __launch_bounds__(256) void place_b_kernel(int64_t n, Dims wzs, Dims ys, uint64_t ny,
                                                       const double *__restrict__ f, const uint8_t *__restrict__ fvalid,
                                                       uint32_t *present, double *b, PlaceFlags *flags) {
@@ -114,6 +116,7 @@ __global__ __launch_bounds__(256) void place_b_kernel(int64_t n, Dims wzs, Dims 
     if (fvalid[r]) {
         const double v = f[r];
         b[p] = v;
+        if (!isfinite(v)) atomicOr(&flags->b_not_finite, 1);
     } else {
         atomicOr(&flags->nulls, 1);
     }

@@ -34,6 +34,7 @@ class Options:
     threads: int = 1
     precision: str = "fp64"
     accept_float32: bool = False
+    fp64_mfma: bool = False
     stream: Optional[int] = None
 
 
@@ -93,6 +94,7 @@ def _options(o: Optional[Options], keep: list) -> _lib.COptions:
     c.domain_rows = 1 if o.unique_domain_keys else 0
     c.precision = _lib.PRECISION[o.precision]
     c.accept_float32 = 1 if o.accept_float32 else 0
+    c.fp64_mfma = 1 if o.fp64_mfma else 0
     c.stream = o.stream
     if o.budget is not None:
         budget = o.budget

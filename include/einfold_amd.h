@@ -194,7 +194,12 @@ typedef struct ef_options {
     uint32_t float_keys;            /* ef_float_keys */
     uint32_t threads;               /* accepted; bits never depend on it */
     uint32_t domain_rows;           /* ef_domain_rows */
-    uint32_t precision;             /* arithmetic for Float32 factors */
+    /* Arithmetic for Float32 factors (with accept_float32).  EF_PRECISION_FP64:
+     * widened to Float64, the reference's exact summation order.  _FP32,
+     * _BF16X3, _BF16: the dense fold's contraction on MFMA with FP32
+     * accumulation (where every factor is finite), the fused pipeline in
+     * that precision. */
+    uint32_t precision;
     /* Budget (options.rs:44-66): grow returns nonzero if `bytes` more may
      * be allocated; shrink gives back bytes grow counted.  NULL: no limit. */
     void *budget_ctx;
@@ -202,7 +207,10 @@ typedef struct ef_options {
     void (*budget_shrink)(void *ctx, size_t bytes);
     void *stream;                   /* hipStream_t; NULL: the default stream */
     uint32_t accept_float32;        /* take Float32 factors (the reference declines them) */
-    uint32_t pad;
+    /* 1: a dense fold of finite Float64 factors may run its contraction on
+     * FP64 MFMA, in another summation order, within the reference's bound
+     * 2 (n - 1) u sum|terms| instead of its exact bits.  0: exact order. */
+    uint32_t fp64_mfma;
 } ef_options;
 
 typedef enum { EF_ALGORITHM_DENSE = 0, EF_ALGORITHM_DISCRIMINATE = 1, EF_ALGORITHM_FUSED = 2 } ef_algorithm;

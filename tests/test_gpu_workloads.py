@@ -183,3 +183,108 @@ def test_device_memory_short_is_out_of_memory_not_a_fault(cuda):
     a, b = data.grid_matmul(32)
     out = kernels.fold_dense(_matmul_fold(), a, b)
     assert out is not None and sum(x.num_rows for x in out) == 32 * 32
+
+
+def _batched_f32(rng, nh, ni, nj, nk, holes, dtype=np.float32):
+    """a(h, i, j, v) with holes and NULL factors, b(h, j, k, v) full."""
+    cells = [(h, i, j) for h in range(nh) for i in range(ni) for j in range(nj)]
+    keep = rng.random(len(cells)) >= holes
+    cells = [c for c, k in zip(cells, keep) if k]
+    av = rng.standard_normal(len(cells)).astype(dtype)
+    a = pa.table({"h": [c[0] for c in cells], "i": [c[1] for c in cells], "j": [c[2] for c in cells],
+                  "v": pa.array(av, mask=rng.random(len(cells)) < 0.05)})
+    hh, jj, kk = np.meshgrid(np.arange(nh), np.arange(nj), np.arange(nk), indexing="ij")
+    b = pa.table({"h": hh.reshape(-1), "j": jj.reshape(-1), "k": kk.reshape(-1),
+                  "v": rng.standard_normal(hh.size).astype(dtype)})
+    return a, b
+
+
+def _batched_fold():
+    # dims 0 = h, 1 = i, 2 = j, 3 = k: SUM(a.v * b.v) GROUP BY h, i, k
+    return ir.Fold([EQ] * 4, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1), ir.Key(2, 2)), 3),
+                              ir.Operand((ir.Key(0, 0), ir.Key(2, 1), ir.Key(3, 2)), 3)],
+                   [ir.Output(0), ir.Output(1), ir.Output(3)])
+
+
+def _magnitudes(a, b, nh, ni, nj, nk):
+    """sum |a b| per output cell over the valued cells."""
+    A = np.zeros((nh, ni, nj))
+    va = a.column(3).to_numpy(zero_copy_only=False).astype(np.float64)
+    ok = ~np.isnan(va) & np.asarray(a.column(3).is_valid())
+    A[a.column(0).to_numpy()[ok], a.column(1).to_numpy()[ok], a.column(2).to_numpy()[ok]] = va[ok]
+    B = b.column(3).to_numpy().astype(np.float64).reshape(nh, nj, nk)
+    return np.abs(A) @ np.abs(B)
+
+
+# Relative bound per precision on top of FP32 accumulation (2 n u32): the
+# split products' own error (BF16x3 recovers about 16 more bits than one
+# BF16 product; BF16 keeps 8).
+PRECISION_BOUND = {"fp32": 0.0, "bf16x3": 2.0 ** -15, "bf16": 2.0 ** -7}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+def test_float32_dense_fold_on_mfma_within_bound(cuda, precision):
+    """Float32 factors with an MFMA precision: the contraction runs as one
+    grouped launch of the W batches' products; keys, groups and NULLs are
+    the oracle's exactly, sums within (c_p + 2 nz u32) sum|terms| of the
+    float64 oracle on the widened tables, c_p the precision's own error."""
+    print(f"float32 dense fold on MFMA {precision}", flush=True)
+    rng = np.random.default_rng(77)
+    nh, ni, nj, nk = 3, 150, 260, 90
+    a, b = _batched_f32(rng, nh, ni, nj, nk, holes=0.1)
+    f = _batched_fold()
+    got = kernels.fold_dense(f, a.to_batches(max_chunksize=20000), b,
+                             options=kernels.Options(accept_float32=True, precision=precision))
+    wa, wb = _widened(a), _widened(b)
+    want = R.fold_dense(f, wa.to_batches(), wb.to_batches(), None, R.schema(f, [wa.schema, wb.schema]))
+    tg, tw = pa.Table.from_batches(got), pa.Table.from_batches(want)
+    assert tg.num_rows == tw.num_rows == nh * ni * nk
+    for col in range(3):
+        assert np.array_equal(tg.column(col).to_numpy(), tw.column(col).to_numpy())
+    assert np.array_equal(np.asarray(tg.column(3).is_null()), np.asarray(tw.column(3).is_null()))
+    mag = _magnitudes(a, b, nh, ni, nj, nk).reshape(-1)
+    g = tg.column(3).to_numpy(zero_copy_only=False)
+    w = tw.column(3).to_numpy(zero_copy_only=False)
+    bound = (PRECISION_BOUND[precision] + 2 * nj * 2.0 ** -24) * mag
+    assert np.all(np.abs(g - w) <= bound), float(np.max(np.abs(g - w) / np.maximum(mag, 1e-300)))
+    # Not the exact order's bits (the products ran on the matrix cores).
+    if precision != "fp32":
+        assert not np.array_equal(g.view(np.uint64), w.view(np.uint64))
+
+
+def test_float32_dense_fold_with_an_infinite_factor_keeps_the_exact_order(cuda):
+    """An infinite factor would turn the matrices' zeros for absent cells
+    into NaN on the matrix cores: such a fold keeps the reference's exact
+    order and its bits."""
+    rng = np.random.default_rng(78)
+    a, b = _batched_f32(rng, 2, 40, 50, 30, holes=0.2)
+    bv = b.column(3).to_numpy().copy()
+    bv[7] = np.inf
+    b = b.set_column(3, "v", pa.array(bv))
+    f = _batched_fold()
+    got = kernels.fold_dense(f, a, b, options=kernels.Options(accept_float32=True, precision="bf16x3"))
+    wa, wb = _widened(a), _widened(b)
+    want = R.fold_dense(f, wa.to_batches(), wb.to_batches(), None, R.schema(f, [wa.schema, wb.schema]))
+    assert_same(got, want)
+
+
+def test_float64_dense_fold_on_fp64_mfma_within_the_reference_bound(cuda):
+    """fp64_mfma: Float64 factors on FP64 MFMA, within the reference's own
+    bound 2 (n - 1) u sum|terms| (kernels_vs_sql.rs:19-23) of its exact
+    order; off by default, where the bits are the exact order's."""
+    print("float64 dense fold on FP64 MFMA", flush=True)
+    rng = np.random.default_rng(79)
+    nh, ni, nj, nk = 2, 130, 300, 70
+    a, b = _batched_f32(rng, nh, ni, nj, nk, holes=0.1, dtype=np.float64)
+    f = _batched_fold()
+    sch = R.schema(f, [a.schema, b.schema])
+    want = R.fold_dense(f, a.to_batches(), b.to_batches(), None, sch)
+    exact = kernels.fold_dense(f, a, b)
+    assert_same(exact, want)
+    got = kernels.fold_dense(f, a, b, options=kernels.Options(fp64_mfma=True))
+    g = pa.Table.from_batches(got).column(3).to_numpy(zero_copy_only=False)
+    w = pa.Table.from_batches(want).column(3).to_numpy(zero_copy_only=False)
+    mag = _magnitudes(a, b, nh, ni, nj, nk).reshape(-1)
+    ok = ~np.isnan(w)
+    assert np.array_equal(ok, ~np.isnan(g))
+    assert np.all(np.abs(g[ok] - w[ok]) <= 2 * (nj - 1) * 2.0 ** -53 * mag[ok])
