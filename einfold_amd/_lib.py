@@ -22,7 +22,7 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
 # Every symbol include/einfold_amd.h declares.
 EXPORTS = [
     "ef_last_error", "ef_version", "ef_init", "ef_result_release",
-    "ef_fold_dense", "ef_fold", "ef_fold_chain", "ef_pipeline",
+    "ef_fold_dense", "ef_fold_dense_many", "ef_fold", "ef_fold_chain", "ef_pipeline",
     "ef_dense_may_run", "ef_dense_run_product", "ef_dense_run_attention",
     "ef_dense_accumulate_f64", "ef_dense_product_f32",
     "ef_rank_scratch_bytes", "ef_rank_u64",
@@ -134,6 +134,12 @@ def lib():
                 f.argtypes = [ctypes.POINTER(desc), ctypes.POINTER(CRelation), ctypes.POINTER(CRelation),
                               ctypes.POINTER(CRelation), ctypes.POINTER(COptions), ctypes.POINTER(CResult)]
                 f.restype = st
+        if hasattr(L, "ef_fold_dense_many"):
+            L.ef_fold_dense_many.argtypes = [sz, ctypes.POINTER(ctypes.POINTER(ir.CFold)), ctypes.POINTER(CRelation),
+                                             ctypes.POINTER(CRelation), ctypes.POINTER(ctypes.POINTER(CRelation)),
+                                             ctypes.POINTER(COptions), ctypes.POINTER(CResult),
+                                             ctypes.POINTER(ctypes.c_int)]
+            L.ef_fold_dense_many.restype = st
         if hasattr(L, "ef_fold_chain"):
             L.ef_fold_chain.argtypes = [ctypes.POINTER(ir.CFold), ctypes.POINTER(CRelation),
                                         ctypes.POINTER(CRelation), ctypes.POINTER(CRelation),

@@ -8,6 +8,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 
 #include "discriminate.h"
@@ -333,9 +334,20 @@ void put_result(std::vector<std::vector<OutColumn>> batches, const std::vector<O
 // The dense positional fold (dense.rs:176-578)
 // ---------------------------------------------------------------------
 
+// A batch of dense folds whose MFMA contractions run as one grouped launch:
+// each fold appends its products and leaves `finish` (the widening of its
+// sums and its emission) to run after that launch.  A fold that leaves
+// `finish` empty has finished on its own.
+struct DenseBatch {
+    std::vector<ef_gemm_problem> problems;
+    std::vector<double> density;
+    uint32_t precision = EF_PRECISION_FP32;
+    std::function<ef_status()> finish;
+};
+
 ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation &b, const Relation *domain,
                           const std::vector<OutField> &fields, Call &c,
-                          std::vector<std::vector<OutColumn>> &result) {
+                          std::vector<std::vector<OutColumn>> &result, DenseBatch *batch) {
     const bool accept32 = c.options && c.options->accept_float32;
     if (m.operands.size() != 2) return EF_DECLINED;
     const OperandModel &A = m.operands[0], &B = m.operands[1];
@@ -545,12 +557,100 @@ ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation 
         }
     }
 
+    double *out;
+    EF_TRY(c.alloc(out, ncell));
+    // Emission (dense.rs:468-577), after the contraction has run: deferred
+    // with it when the fold is one of a batch whose MFMA products share a
+    // launch.  It holds the call's device buffers by value.
+    auto finish_fold = [=, &c, &result, &m, &fields]() -> ef_status {
+        if (domain) {
+            // dense.rs:488-515: each domain row gets its cell's value where the
+            // group exists (and has a value), else the default or NULL.
+            for (size_t bi = 0; bi < domain->batches.size(); bi++) {
+                const int64_t n = domain->batch_rows[bi];
+                double *vals;
+                uint8_t *valid;
+                EF_TRY(c.alloc(vals, (size_t)n));
+                EF_TRY(c.alloc(valid, (size_t)n));
+                EF_TRY(domain_values(n, cells[bi], ny, out, zero_default ? nullptr : valued, conds, m.has_missing,
+                                     m.missing, vals, valid, c.stream));
+                std::vector<OutColumn> cols;
+                for (size_t i = 0; i < m.carry.size(); i++)
+                    cols.push_back(copy_column(domain->batches[bi][m.carry[i]], n, fields[i].nullable));
+                OutColumn v;
+                EF_TRY(fetch_column(c, Type::Float64, vals, m.has_missing ? nullptr : valid, n, fields.back().nullable, v));
+                cols.push_back(std::move(v));
+                result.push_back(std::move(cols));
+            }
+            return EF_OK;
+        }
+        // dense.rs:516-577: every cell of every reached row, keys recomputed
+        // from positions, filtered by the conditions.
+        uint32_t *tmp32, *excl, *rows;
+        void *scan_tmp;
+        uint8_t *scan_bytes;
+        EF_TRY(c.alloc(tmp32, nx));
+        EF_TRY(c.alloc(excl, nx));
+        EF_TRY(c.alloc(rows, nx));
+        EF_TRY(c.alloc(scan_bytes, scan_scratch_bytes(std::max<uint64_t>(nx, ncell))));
+        scan_tmp = scan_bytes;
+        EF_TRY(reached_rows((int64_t)nx, reached, tmp32, excl, rows, scan_tmp, c.stream));
+        uint32_t last[2] = {0, 0};
+        uint8_t last_reached = 0;
+        if (nx) {
+            EF_TRY(c.download<uint32_t>(&last[0], excl + nx - 1, 1));
+            EF_TRY(c.download<uint8_t>(&last_reached, reached + nx - 1, 1));
+        }
+        EF_TRY(c.sync());
+        const uint64_t n_rows = nx ? (uint64_t)last[0] + last_reached : 0;
+        const uint64_t n = n_rows * ny;
+        uint32_t *keep = nullptr, *kexcl = nullptr;
+        uint64_t kept = n;
+        if (!m.conditions.empty() && n) {
+            EF_TRY(c.alloc(keep, n));
+            EF_TRY(c.alloc(kexcl, n));
+            EF_TRY(keep_cells((int64_t)n, ny, rows, conds, keep, c.stream));
+            EF_TRY(scan_u32(keep, kexcl, n, scan_tmp, c.stream));
+            uint32_t lk[2];
+            EF_TRY(c.download<uint32_t>(&lk[0], kexcl + n - 1, 1));
+            EF_TRY(c.download<uint32_t>(&lk[1], keep + n - 1, 1));
+            EF_TRY(c.sync());
+            kept = (uint64_t)lk[0] + lk[1];
+        }
+        OutKeys ok{};
+        ok.n = (int)m.outputs.size();
+        std::vector<uint8_t *> key_out(m.outputs.size());
+        for (size_t i = 0; i < m.outputs.size(); i++) {
+            const uint32_t d = m.outputs[i].dimension;
+            const int w = width_of(fields[i].type);
+            EF_TRY(c.alloc(key_out[i], kept * (w ? w : 8)));
+            ok.dst[i] = key_out[i];
+            ok.width[i] = w;
+            ok.in_x[i] = index_in(xs, d) >= 0;
+            ok.stride[i] = stride_in(ok.in_x[i] ? xs : ys, d, extent);
+            ok.extent[i] = extent[d] ? extent[d] : 1;
+        }
+        double *vals;
+        uint8_t *valid;
+        EF_TRY(c.alloc(vals, kept));
+        EF_TRY(c.alloc(valid, kept));
+        EF_TRY(emit((int64_t)n, ny, rows, keep, kexcl, ok, out, valued, vals, valid, c.stream));
+        std::vector<OutColumn> cols;
+        for (size_t i = 0; i < m.outputs.size(); i++) {
+            OutColumn k;
+            EF_TRY(fetch_column(c, fields[i].type, key_out[i], nullptr, (int64_t)kept, fields[i].nullable, k));
+            cols.push_back(std::move(k));
+        }
+        OutColumn v;
+        EF_TRY(fetch_column(c, Type::Float64, vals, valid, (int64_t)kept, true, v));
+        cols.push_back(std::move(v));
+        result.push_back(std::move(cols));
+        return EF_OK;
+    };
     // The contraction (dense.rs:706-770): in the reference's order, or on
     // MFMA where the caller asked for it and every factor is finite (the
     // matrices hold 0 for absent and NULL cells, which then add nothing;
     // with an infinite factor, 0 * inf would).
-    double *out;
-    EF_TRY(c.alloc(out, ncell));
     const uint64_t pb = per_batch ? per_batch : 1;
     const bool all_finite = finite && !pf.b_not_finite;
     const uint32_t prec = c.options ? c.options->precision : EF_PRECISION_FP64;
@@ -570,9 +670,21 @@ ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation 
         const double density = std::min((double)a.rows / (double)std::max<uint64_t>(na, 1),
                                         (double)b.rows / (double)std::max<uint64_t>(nb, 1));
         std::vector<ef_gemm_problem> probs(nw);
-        std::vector<double> dens(nw, density);
         for (uint64_t w = 0; w < nw; w++)
             probs[w] = {a32 + w * pb * nz, b32 + w * nz * ny, c32 + w * pb * ny, (size_t)pb, (size_t)ny, (size_t)nz};
+        if (batch) {
+            // The products join the batch's one launch; the widening and
+            // the emission wait for it.
+            batch->problems.insert(batch->problems.end(), probs.begin(), probs.end());
+            batch->density.insert(batch->density.end(), probs.size(), density);
+            batch->precision = prec;
+            batch->finish = [=, &c]() -> ef_status {
+                EF_TRY(to_f64(Type::Float32, c32, (int64_t)ncell, out, c.stream));
+                return finish_fold();
+            };
+            return EF_OK;
+        }
+        std::vector<double> dens(nw, density);
         EF_TRY(grouped_run_once(probs.data(), probs.size(), prec, dens.data(), c.stream));
         EF_TRY(to_f64(Type::Float32, c32, (int64_t)ncell, out, c.stream));
     } else if (!f32 && c.options && c.options->fp64_mfma && all_finite && nw <= 65535) {
@@ -583,89 +695,7 @@ ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation 
         EF_TRY(launch_dense_accumulate_f64(am, held, bm, nx, nz, ny, pb, 1, out, c.stream));
     }
 
-    if (domain) {
-        // dense.rs:488-515: each domain row gets its cell's value where the
-        // group exists (and has a value), else the default or NULL.
-        for (size_t bi = 0; bi < domain->batches.size(); bi++) {
-            const int64_t n = domain->batch_rows[bi];
-            double *vals;
-            uint8_t *valid;
-            EF_TRY(c.alloc(vals, (size_t)n));
-            EF_TRY(c.alloc(valid, (size_t)n));
-            EF_TRY(domain_values(n, cells[bi], ny, out, zero_default ? nullptr : valued, conds, m.has_missing,
-                                 m.missing, vals, valid, c.stream));
-            std::vector<OutColumn> cols;
-            for (size_t i = 0; i < m.carry.size(); i++)
-                cols.push_back(copy_column(domain->batches[bi][m.carry[i]], n, fields[i].nullable));
-            OutColumn v;
-            EF_TRY(fetch_column(c, Type::Float64, vals, m.has_missing ? nullptr : valid, n, fields.back().nullable, v));
-            cols.push_back(std::move(v));
-            result.push_back(std::move(cols));
-        }
-        return EF_OK;
-    }
-    // dense.rs:516-577: every cell of every reached row, keys recomputed
-    // from positions, filtered by the conditions.
-    uint32_t *tmp32, *excl, *rows;
-    void *scan_tmp;
-    uint8_t *scan_bytes;
-    EF_TRY(c.alloc(tmp32, nx));
-    EF_TRY(c.alloc(excl, nx));
-    EF_TRY(c.alloc(rows, nx));
-    EF_TRY(c.alloc(scan_bytes, scan_scratch_bytes(std::max<uint64_t>(nx, ncell))));
-    scan_tmp = scan_bytes;
-    EF_TRY(reached_rows((int64_t)nx, reached, tmp32, excl, rows, scan_tmp, c.stream));
-    uint32_t last[2] = {0, 0};
-    uint8_t last_reached = 0;
-    if (nx) {
-        EF_TRY(c.download<uint32_t>(&last[0], excl + nx - 1, 1));
-        EF_TRY(c.download<uint8_t>(&last_reached, reached + nx - 1, 1));
-    }
-    EF_TRY(c.sync());
-    const uint64_t n_rows = nx ? (uint64_t)last[0] + last_reached : 0;
-    const uint64_t n = n_rows * ny;
-    uint32_t *keep = nullptr, *kexcl = nullptr;
-    uint64_t kept = n;
-    if (!m.conditions.empty() && n) {
-        EF_TRY(c.alloc(keep, n));
-        EF_TRY(c.alloc(kexcl, n));
-        EF_TRY(keep_cells((int64_t)n, ny, rows, conds, keep, c.stream));
-        EF_TRY(scan_u32(keep, kexcl, n, scan_tmp, c.stream));
-        uint32_t lk[2];
-        EF_TRY(c.download<uint32_t>(&lk[0], kexcl + n - 1, 1));
-        EF_TRY(c.download<uint32_t>(&lk[1], keep + n - 1, 1));
-        EF_TRY(c.sync());
-        kept = (uint64_t)lk[0] + lk[1];
-    }
-    OutKeys ok{};
-    ok.n = (int)m.outputs.size();
-    std::vector<uint8_t *> key_out(m.outputs.size());
-    for (size_t i = 0; i < m.outputs.size(); i++) {
-        const uint32_t d = m.outputs[i].dimension;
-        const int w = width_of(fields[i].type);
-        EF_TRY(c.alloc(key_out[i], kept * (w ? w : 8)));
-        ok.dst[i] = key_out[i];
-        ok.width[i] = w;
-        ok.in_x[i] = index_in(xs, d) >= 0;
-        ok.stride[i] = stride_in(ok.in_x[i] ? xs : ys, d, extent);
-        ok.extent[i] = extent[d] ? extent[d] : 1;
-    }
-    double *vals;
-    uint8_t *valid;
-    EF_TRY(c.alloc(vals, kept));
-    EF_TRY(c.alloc(valid, kept));
-    EF_TRY(emit((int64_t)n, ny, rows, keep, kexcl, ok, out, valued, vals, valid, c.stream));
-    std::vector<OutColumn> cols;
-    for (size_t i = 0; i < m.outputs.size(); i++) {
-        OutColumn k;
-        EF_TRY(fetch_column(c, fields[i].type, key_out[i], nullptr, (int64_t)kept, fields[i].nullable, k));
-        cols.push_back(std::move(k));
-    }
-    OutColumn v;
-    EF_TRY(fetch_column(c, Type::Float64, vals, valid, (int64_t)kept, true, v));
-    cols.push_back(std::move(v));
-    result.push_back(std::move(cols));
-    return EF_OK;
+    return finish_fold();
 }
 
 // ---------------------------------------------------------------------
@@ -1723,7 +1753,7 @@ ef_status chain_either(const FoldModel &f, const Relation &a, const Relation &b,
                        const ef_options *options, std::vector<std::vector<OutColumn>> &out) {
     {
         Call c(options);
-        const ef_status st = fold_dense_impl(f, a, b, nullptr, fields, c, out);
+        const ef_status st = fold_dense_impl(f, a, b, nullptr, fields, c, out, nullptr);
         (void)hipStreamSynchronize(c.stream);
         if (st != EF_DECLINED) return st;
     }
@@ -1731,7 +1761,7 @@ ef_status chain_either(const FoldModel &f, const Relation &a, const Relation &b,
     std::swap(swapped.operands[0], swapped.operands[1]);
     out.clear();
     Call c(options);
-    const ef_status st = fold_dense_impl(swapped, b, a, nullptr, fields, c, out);
+    const ef_status st = fold_dense_impl(swapped, b, a, nullptr, fields, c, out, nullptr);
     (void)hipStreamSynchronize(c.stream);
     return st;
 }
@@ -1855,12 +1885,79 @@ ef_status ef_fold_dense(const ef_fold_desc *fold, const ef_relation *a, const ef
         EF_TRY(prepare(m, a, b, domain, options, ra, rb, rd, has_domain, fields));
         Call c(options);
         std::vector<std::vector<OutColumn>> batches;
-        ef_status st = fold_dense_impl(m, ra, rb, has_domain ? &rd : nullptr, fields, c, batches);
+        ef_status st = fold_dense_impl(m, ra, rb, has_domain ? &rd : nullptr, fields, c, batches, nullptr);
         if (st != EF_OK) {
             (void)hipStreamSynchronize(c.stream);
             return st;
         }
         put_result(std::move(batches), fields, EF_ALGORITHM_DENSE, out);
+        return EF_OK;
+    });
+}
+
+ef_status ef_fold_dense_many(size_t count, const ef_fold_desc *const *folds, const ef_relation *a,
+                             const ef_relation *b, const ef_relation *const *domains, const ef_options *options,
+                             ef_result *out, ef_status *status) {
+    clear_error();
+    if (count && (!folds || !a || !b || !out || !status)) {
+        set_error("folds, operands, results and statuses are required");
+        return EF_INVALID;
+    }
+    for (size_t i = 0; i < count; i++) {
+        std::memset(&out[i], 0, sizeof out[i]);
+        status[i] = EF_DECLINED;
+    }
+    return guarded([&]() -> ef_status {
+        // Every fold is read, its keys cast and its operands placed; the
+        // MFMA contractions wait in one batch, then run as one launch; then
+        // each deferred fold widens its sums and emits.
+        struct One {
+            FoldModel m;
+            Relation ra, rb, rd;
+            bool has_domain = false;
+            std::vector<OutField> fields;
+            std::vector<std::vector<OutColumn>> batches;
+            std::function<ef_status()> finish;
+        };
+        std::deque<One> ones(count);
+        Call c(options);
+        DenseBatch batch;
+        for (size_t i = 0; i < count; i++) {
+            One &o = ones[i];
+            if (!parse_fold(folds[i], o.m)) {
+                status[i] = EF_INVALID;
+                continue;
+            }
+            ef_status st = prepare(o.m, &a[i], &b[i], domains ? domains[i] : nullptr, options, o.ra, o.rb, o.rd,
+                                   o.has_domain, o.fields);
+            if (st == EF_OK) {
+                batch.finish = nullptr;
+                st = fold_dense_impl(o.m, o.ra, o.rb, o.has_domain ? &o.rd : nullptr, o.fields, c, o.batches, &batch);
+                o.finish = std::move(batch.finish);
+                batch.finish = nullptr;
+            }
+            if (st == EF_DEVICE_ERROR) {
+                (void)hipStreamSynchronize(c.stream);
+                return st;   // a device failure ends the call
+            }
+            status[i] = st;
+        }
+        if (!batch.problems.empty())
+            EF_TRY(grouped_run_once(batch.problems.data(), batch.problems.size(), batch.precision,
+                                    batch.density.data(), c.stream));
+        for (size_t i = 0; i < count; i++) {
+            One &o = ones[i];
+            if (status[i] != EF_OK) continue;
+            if (o.finish) {
+                const ef_status st = o.finish();
+                if (st != EF_OK) {
+                    (void)hipStreamSynchronize(c.stream);
+                    return st;
+                }
+            }
+            put_result(std::move(o.batches), o.fields, EF_ALGORITHM_DENSE, &out[i]);
+        }
+        EF_HIP(hipStreamSynchronize(c.stream));
         return EF_OK;
     });
 }
@@ -1891,7 +1988,7 @@ ef_status ef_fold(const ef_fold_desc *fold, const ef_relation *a, const ef_relat
         std::vector<std::vector<OutColumn>> batches;
         {
             Call c(options);
-            ef_status st = fold_dense_impl(m, ra, rb, has_domain ? &rd : nullptr, fields, c, batches);
+            ef_status st = fold_dense_impl(m, ra, rb, has_domain ? &rd : nullptr, fields, c, batches, nullptr);
             (void)hipStreamSynchronize(c.stream);
             if (st == EF_OK) {
                 put_result(std::move(batches), fields, EF_ALGORITHM_DENSE, out);

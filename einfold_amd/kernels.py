@@ -149,6 +149,54 @@ def fold_dense(fold: ir.Fold, a: RelationLike, b: RelationLike, domain: Optional
     return None if r is None else r[0]
 
 
+def fold_dense_many(folds: Sequence[ir.Fold], a: Sequence[RelationLike], b: Sequence[RelationLike],
+                    domains: Optional[Sequence[Optional[RelationLike]]] = None,
+                    options: Optional[Options] = None) -> List[Optional[List[pa.RecordBatch]]]:
+    """Many dense folds in one call (ef_fold_dense_many): each fold's result
+    as fold_dense would give it (None for a decline), their MFMA
+    contractions run as one grouped launch.  A fold whose budget is refused
+    raises MemoryError; an invalid one ValueError."""
+    n = len(folds)
+    if len(a) != n or len(b) != n or (domains is not None and len(domains) != n):
+        raise ValueError("one a, one b (and one domain) per fold")
+    keep: list = []
+    descs = [ir.fold_to_c(f, keep) for f in folds]
+    dptrs = (ctypes.POINTER(ir.CFold) * max(1, n))(*[ctypes.pointer(d) for d in descs])
+    ea = [_Exported(r) for r in a]
+    eb = [_Exported(r) for r in b]
+    ed = [None if d is None else _Exported(d) for d in (domains or [None] * n)]
+    ca = (_lib.CRelation * max(1, n))(*[e.c for e in ea])
+    cb = (_lib.CRelation * max(1, n))(*[e.c for e in eb])
+    cd = (ctypes.POINTER(_lib.CRelation) * max(1, n))(*[None if e is None else ctypes.pointer(e.c) for e in ed])
+    opts = _options(options, keep)
+    res = (_lib.CResult * max(1, n))()
+    status = (ctypes.c_int * max(1, n))()
+    try:
+        st = _lib.lib().ef_fold_dense_many(n, dptrs, ca, cb, cd, ctypes.byref(opts), res, status)
+    finally:
+        for e in ea + eb + [x for x in ed if x is not None]:
+            e.release()
+    _lib.check(st)
+    out: List[Optional[List[pa.RecordBatch]]] = []
+    try:
+        for i in range(n):
+            if status[i] == _lib.DECLINED:
+                out.append(None)
+            elif status[i] == _lib.OUT_OF_MEMORY:
+                raise MemoryError(f"fold {i}: budget refused")
+            elif status[i] == _lib.INVALID:
+                raise ValueError(f"fold {i}: invalid")
+            else:
+                _lib.check(status[i])
+                schema = pa.Schema._import_from_c(ctypes.addressof(res[i].schema))
+                out.append([pa.RecordBatch._import_from_c(ctypes.addressof(res[i].batches[j]), schema)
+                            for j in range(res[i].n_batches)])
+    finally:
+        for i in range(n):
+            _lib.lib().ef_result_release(ctypes.byref(res[i]))
+    return out
+
+
 def fold(fold: ir.Fold, a: RelationLike, b: RelationLike, domain: Optional[RelationLike] = None,
          options: Optional[Options] = None) -> Folded:
     """algorithm.rs:97-133: dense, else discrimination."""

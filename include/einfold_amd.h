@@ -239,6 +239,19 @@ void ef_result_release(ef_result *result);
 ef_status ef_fold_dense(const ef_fold_desc *fold, const ef_relation *a, const ef_relation *b,
                         const ef_relation *domain, const ef_options *options, ef_result *out);
 
+/* Many dense folds in one call, each as ef_fold_dense would compute it
+ * (algorithm.rs:52-62; EinFoldExec calls fold_dense once per fold,
+ * ein_fold_exec.rs:674-727): every fold's keys are cast and its operands
+ * placed, then every contraction that runs on MFMA (Float32 factors with an
+ * MFMA precision) joins one persistent grouped launch, then each fold
+ * emits.  a, b: `count` relations each; domains: `count` pointers (NULL
+ * for none) or NULL.  status[i] and out[i] are fold i's status and result
+ * (EF_DECLINED, EF_OUT_OF_MEMORY and EF_INVALID per fold).  Returns EF_OK
+ * unless the device failed. */
+ef_status ef_fold_dense_many(size_t count, const ef_fold_desc *const *folds, const ef_relation *a,
+                             const ef_relation *b, const ef_relation *const *domains, const ef_options *options,
+                             ef_result *out, ef_status *status);
+
 /* fold (algorithm.rs:97-133): dense, else discrimination (radix ranking,
  * counting sorts, Gustavson's product: discriminate.rs:369-650). */
 ef_status ef_fold(const ef_fold_desc *fold, const ef_relation *a, const ef_relation *b,

@@ -288,3 +288,72 @@ def test_float64_dense_fold_on_fp64_mfma_within_the_reference_bound(cuda):
     ok = ~np.isnan(w)
     assert np.array_equal(ok, ~np.isnan(g))
     assert np.all(np.abs(g[ok] - w[ok]) <= 2 * (nj - 1) * 2.0 ** -53 * mag[ok])
+
+
+def _grid32(m, k, seed, salt, names, dtype=np.float32):
+    v = (data.unit(seed, salt, m * k) * 2 - 1).astype(dtype)
+    return pa.table({names[0]: np.repeat(np.arange(m, dtype=np.int64), k),
+                     names[1]: np.tile(np.arange(k, dtype=np.int64), m), names[2]: v})
+
+
+def test_fold_dense_many_one_launch_matches_each_fold(cuda):
+    """Uneven Float32 folds (config 5's batching, M, N, K log-uniform in
+    [32, 512]) in one ef_fold_dense_many call: each result within the
+    BF16x3 bound of the float64 oracle on its widened tables; a Float64 fold
+    in the same call keeps the exact order (bit-exact); a fold with a
+    negative key declines alone; a fold with a domain left-joins."""
+    print("fold_dense_many: 24 uneven folds + 3 special", flush=True)
+    shapes = data.log_uniform_shapes(24, 32, 512, seed=12)
+    f = _matmul_fold()
+    folds, As, Bs, doms = [], [], [], []
+    for n, (m, nn, k) in enumerate(shapes):
+        As.append(_grid32(m, k, 12, 2 * n, "ijv"))
+        Bs.append(_grid32(k, nn, 12, 2 * n + 1, "jkv"))
+        folds.append(f)
+        doms.append(None)
+    # Float64 factors: the exact order, inside the same call.
+    As.append(_grid32(70, 90, 13, 1, "ijv", np.float64))
+    Bs.append(_grid32(90, 60, 13, 2, "jkv", np.float64))
+    folds.append(f)
+    doms.append(None)
+    # A negative key: declines.
+    neg = _grid32(10, 10, 14, 1, "ijv")
+    As.append(neg.set_column(0, "i", pa.array(neg.column(0).to_numpy() - 1)))
+    Bs.append(_grid32(10, 10, 14, 2, "jkv"))
+    folds.append(f)
+    doms.append(None)
+    # A domain (missing 0.0) on Float32 factors.
+    As.append(_grid32(40, 30, 15, 1, "ijv"))
+    Bs.append(_grid32(30, 20, 15, 2, "jkv"))
+    folds.append(ir.Fold(f.dimensions, f.operands, [ir.Output(0, ir.DomainKey(0)), ir.Output(2, ir.DomainKey(1))],
+                         domain=ir.Domain((0, 1), 0.0)))
+    doms.append(pa.table({"x": pa.array([0, 39, 40, -1, 5], pa.int64()), "y": pa.array([0, 19, 3, 3, 21], pa.int64())}))
+    opts = kernels.Options(accept_float32=True, precision="bf16x3")
+    got = kernels.fold_dense_many(folds, As, Bs, doms, opts)
+    assert len(got) == len(folds)
+    for n in range(len(folds)):
+        wa, wb = _widened(As[n]), _widened(Bs[n])
+        wd = doms[n]
+        want = R.fold_dense(folds[n], wa.to_batches(), wb.to_batches(), None if wd is None else wd.to_batches(),
+                            R.schema(folds[n], [wa.schema, wb.schema], None if wd is None else wd.schema))
+        if want is None:
+            assert got[n] is None and n == len(shapes) + 1
+            continue
+        if n == len(shapes):
+            assert_same(got[n], want)
+            continue
+        tg, tw = pa.Table.from_batches(got[n]), pa.Table.from_batches(want)
+        assert tg.num_rows == tw.num_rows
+        for col in range(tw.num_columns - 1):
+            assert np.array_equal(tg.column(col).to_numpy(), tw.column(col).to_numpy())
+        g = tg.column(tg.num_columns - 1).to_numpy()
+        w = tw.column(tw.num_columns - 1).to_numpy()
+        k = As[n].column(1).to_numpy().max() + 1
+        # |values| < 1, so sum|terms| < k bounds each sum's magnitude.
+        assert np.all(np.abs(g - w) <= (2.0 ** -15 + 2 * k * 2.0 ** -24) * k), n
+    # The same folds one at a time give the same keys and sums within the bound.
+    one = kernels.fold_dense(folds[3], As[3], Bs[3], options=opts)
+    g1 = pa.Table.from_batches(one).column(2).to_numpy()
+    gm = pa.Table.from_batches(got[3]).column(2).to_numpy()
+    k3 = shapes[3][2]
+    assert np.all(np.abs(g1 - gm) <= 2 * (2.0 ** -15 + 2 * k3 * 2.0 ** -24) * k3)
