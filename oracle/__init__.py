@@ -55,6 +55,52 @@ def lib():
     return _lib
 
 
+_cpu = None
+
+
+def cpu_lib():
+    """The CPU baseline's library: the AVX2 + FMA build where the host has
+    both, else the generic one."""
+    global _cpu
+    if _cpu is None:
+        flags = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("flags"):
+                        flags = line
+                        break
+        except OSError:
+            pass
+        fast = " avx2" in flags and " fma" in flags
+        path = os.path.join(HERE, "_build", "libef_cpu_avx2.so" if fast else "libef_cpu_generic.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P, sz = ctypes.c_void_p, ctypes.c_size_t
+        L.ef_cpu_attention_fwd.argtypes = [P, P, P, sz, sz, sz, sz, ctypes.c_double, P, P, P, ctypes.c_int]
+        L.ef_cpu_attention_fwd.restype = None
+        _cpu = (L, "avx2+fma" if fast else "generic x86-64")
+    return _cpu
+
+
+def cpu_attention_fwd(q, kt, v, scale, lo, hi, threads):
+    """The CPU baseline's fused attention of one head (ef_cpu_baseline.c):
+    q [nx][ne], kt [ne][nj], v [nj][ny], each query's keys [lo, hi)."""
+    L, _ = cpu_lib()
+    q = np.ascontiguousarray(q, np.float64)
+    kt = np.ascontiguousarray(kt, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    lo = np.ascontiguousarray(lo, np.uint64)
+    hi = np.ascontiguousarray(hi, np.uint64)
+    nx, ne = q.shape
+    nj, ny = v.shape
+    out = np.empty((nx, ny), np.float64)
+    L.ef_cpu_attention_fwd(ptr(q), ptr(kt), ptr(v), nx, ne, nj, ny, float(scale), ptr(lo), ptr(hi), ptr(out),
+                           int(threads))
+    return out
+
+
 def ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 

@@ -477,3 +477,22 @@ def test_expression_known_answers():
     over0, _ = _oracle_expr(E.lit(1.0) / 0.0, lambda o, c: (0, 0))
     fn(ctypes.cast(over0, ctypes.c_void_p), 3, oracle.ptr(vals), 1, oracle.ptr(res))
     assert res[0] == math.inf
+
+
+@pytest.mark.parametrize("T,window,ne,ny", [(300, 0, 24, 40), (257, 31, 64, 64), (64, 0, 7, 3)])
+def test_cpu_baseline_matches_float64_softmax_at_any_thread_count(T, window, ne, ny):
+    """The bench's CPU baseline (a fast restatement of the fused tile loop)
+    gives the float64 softmax within 1e-12 * max|v|, the same bits on 1 and
+    5 threads (tasks own their rows), and NaN-free rows."""
+    rng = np.random.default_rng(T + ne)
+    q = rng.standard_normal((T, ne)) * 0.3
+    k = rng.standard_normal((T, ne)) * 0.3
+    v = rng.standard_normal((T, ny))
+    x = np.arange(T)
+    hi = x + 1
+    lo = np.maximum(0, x - window + 1) if window else np.zeros(T, np.int64)
+    one = oracle.cpu_attention_fwd(q, np.ascontiguousarray(k.T), v, 0.7, lo, hi, 1)
+    five = oracle.cpu_attention_fwd(q, np.ascontiguousarray(k.T), v, 0.7, lo, hi, 5)
+    assert np.array_equal(one.view(np.uint64), five.view(np.uint64))
+    want = _np_attention(q, k, v, 0.7, lo, hi)
+    assert np.max(np.abs(one - want)) <= 1e-12 * np.abs(v).max()
