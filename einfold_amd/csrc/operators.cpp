@@ -1148,6 +1148,55 @@ ef_status cast_operand_keys(Call &c, KeyedOperand &ko, size_t n_dims, KeyStats *
 // Whether a postfix program is well formed for the device evaluator: every
 // operator finds its operands, the stack never holds more than 16 values,
 // and one value is left.
+// Whether a score program is a squared difference of one q-side and one
+// k-side column, (c0 - c1)^2, followed by up to four operations with a
+// literal (negation; +, -, * or / by a literal on either side), and its
+// parts in `pa`: the kernels then evaluate it without the interpreter, the
+// same operations in the same order.
+bool squared_difference(const std::vector<ef_expr_op> &e, PipeArgs &pa) {
+    if (e.size() < 5 || e[0].code != EF_EXPR_COLUMN || e[1].code != EF_EXPR_COLUMN || e[2].code != EF_EXPR_SUB ||
+        e[3].code != EF_EXPR_LITERAL || e[3].value != 2.0 || e[4].code != EF_EXPR_POW)
+        return false;
+    if (e[0].operand == e[1].operand || e[0].operand > 1 || e[1].operand > 1) return false;
+    int n = 0;
+    for (size_t i = 5; i < e.size();) {
+        if (n == 4) return false;
+        if (e[i].code == EF_EXPR_NEG) {
+            pa.post_code[n] = EF_EXPR_NEG;
+            pa.post_value[n] = 0.0;
+            pa.post_literal_first[n] = 0;
+            n++;
+            i++;
+            continue;
+        }
+        if (i + 1 < e.size() && e[i].code == EF_EXPR_LITERAL &&
+            (e[i + 1].code == EF_EXPR_MUL || e[i + 1].code == EF_EXPR_DIV || e[i + 1].code == EF_EXPR_ADD ||
+             e[i + 1].code == EF_EXPR_SUB)) {
+            pa.post_code[n] = e[i + 1].code;
+            pa.post_value[n] = e[i].value;
+            pa.post_literal_first[n] = 0;
+            n++;
+            i += 2;
+            continue;
+        }
+        return false;
+    }
+    pa.n_post = n;
+    // Monotone in the square: finite literals, no product or quotient by 0
+    // (and a finite, nonzero weight scale, checked by the caller).
+    pa.sq_mono = 1;
+    for (int i = 0; i < n; i++) {
+        const double c = pa.post_value[i];
+        if (!std::isfinite(c) || ((pa.post_code[i] == EF_EXPR_MUL || pa.post_code[i] == EF_EXPR_DIV) && c == 0.0))
+            pa.sq_mono = 0;
+    }
+    pa.sq_neg_div = n == 2 && pa.post_code[0] == EF_EXPR_NEG && pa.post_code[1] == EF_EXPR_DIV;
+    pa.sq_swap = e[0].operand == 1 ? 1 : 0;
+    pa.sq_q = (int)(e[0].operand == 0 ? e[0].column : e[1].column);
+    pa.sq_k = (int)(e[0].operand == 1 ? e[0].column : e[1].column);
+    return true;
+}
+
 bool program_ok(const ef_expr_op *ops, size_t n) {
     int depth = 0;
     for (size_t i = 0; i < n; i++) {
@@ -1487,6 +1536,9 @@ ef_status pipeline_impl(const ef_pipeline_desc *pd, const FoldModel &prod, const
         }
         pa.n_q_cols = nq_cols;
         pa.n_k_cols = nk_cols;
+        pa.sq_q = -1;
+        if (!score.empty() && !squared_difference(score, pa)) pa.sq_q = -1;
+        if (pd->scale_kind != EF_SCALE_ONE && (!std::isfinite(pd->scale) || pd->scale == 0.0)) pa.sq_mono = 0;
         ef_expr_op *dscore = nullptr, *dweight = nullptr;
         if (!score.empty()) {
             EF_TRY(c.alloc(dscore, score.size()));
@@ -1519,10 +1571,34 @@ ef_status pipeline_impl(const ef_pipeline_desc *pd, const FoldModel &prod, const
         pa.has_inf = has_inf;
         pa.largest = largest;
         pa.smallest = smallest;
+        {
+            const uint64_t tiles = (nj + 63) / 64;
+            uint8_t *tf;
+            double *tl, *ts;
+            EF_TRY(c.alloc(tf, tiles));
+            EF_TRY(c.alloc(tl, tiles));
+            EF_TRY(c.alloc(ts, tiles));
+            EF_TRY(tile_stats(finite, largest, smallest, (int64_t)nj, tf, tl, ts, c.stream));
+            pa.tile_finite = tf;
+            pa.tile_largest = tl;
+            pa.tile_smallest = ts;
+        }
         pa.out = out;
         EF_TRY(c.alloc(pa.special, nx * ny));
         pa.overflowed = bad + 3;
-        EF_TRY(pipeline_rows(pa, c.stream));
+        // Rows and their tiles over the CUs: with few rows, each row's tiles
+        // split into runs (parts) so that about four waves run per SIMD,
+        // merged in run order; the count depends only on the shape.
+        int64_t most_tiles = 0;
+        for (uint64_t x = 0; x < nx; x++)
+            if (lo[x] < hi[x]) most_tiles = std::max<int64_t>(most_tiles, (hi[x] - 1) / 64 + 1 - lo[x] / 64);
+        const int64_t want = 4 * 1024 * 64;
+        int parts = (int)std::min<int64_t>(std::min<int64_t>((want + (int64_t)nx - 1) / std::max<int64_t>(nx, 1), 64),
+                                           std::max<int64_t>(most_tiles, 1));
+        if (parts < 2) parts = 1;
+        uint8_t *parts_scratch = nullptr;
+        if (parts > 1) EF_TRY(c.alloc(parts_scratch, pipeline_parts_bytes(pa, parts)));
+        EF_TRY(pipeline_rows(pa, parts, parts_scratch, c.stream));
         int ov_flag = 0;
         EF_TRY(c.download<int>(&ov_flag, bad + 3, 1));
         EF_TRY(c.sync());

@@ -238,3 +238,44 @@ def test_pipeline_declines_duplicates_and_holes(cuda):
     p = attention_pipeline([(GE, 0)], False)
     assert kernels.pipeline(p, pa.concat_tables([q, q.slice(0, 1)]), k, v) is None
     assert kernels.pipeline(p, q.slice(1), k, v) is None
+
+
+def test_smoother_rows_split_into_parts_against_numpy(cuda):
+    """2000 query rows over 5000 keys: each row's 79 key tiles split into
+    64 runs merged in run order; within 1e-12 of the float64 formula and of
+    the oracle's reference-order sums (the squared-difference producer is
+    evaluated without the interpreter, the same operations in order)."""
+    print("smoother 2000 x 5000 in parts", flush=True)
+    rng = np.random.default_rng(29)
+    n, m, h = 2000, 5000, 4.0
+    xv, yv, wv = rng.uniform(0, 100, n), rng.uniform(0, 100, m), rng.normal(size=m)
+    xs = pa.table({"i": np.arange(n, dtype=np.int64), "x": xv})
+    ys = pa.table({"j": np.arange(m, dtype=np.int64), "y": yv})
+    ws = pa.table({"j": np.arange(m, dtype=np.int64), "w": wv})
+    p = smoother(n, m, h)
+    got, want = both(p, xs, ys, ws)
+    compare(got, want, 1e-12 * np.max(np.abs(wv)))
+    s = -((xv[:, None] - yv[None, :]) ** 2) / h
+    w = np.exp(s - s.max(axis=1, keepdims=True))
+    ref = (w @ wv) / w.sum(axis=1)
+    g = pa.Table.from_batches(got.batches).column(1).to_numpy()
+    assert np.max(np.abs(g - ref)) <= 1e-12 * np.max(np.abs(wv))
+
+
+def test_general_path_many_rows_one_run_each(cuda):
+    """270k query rows (enough to fill the chip one row per thread: no
+    split), Weight::Of, against the oracle."""
+    print("general path 270k rows", flush=True)
+    rng = np.random.default_rng(30)
+    nq, nk, d = 270000, 70, 4
+
+    def grid(rows, cols, vals):
+        r, c = np.meshgrid(np.arange(rows, dtype=np.int64), np.arange(cols, dtype=np.int64), indexing="ij")
+        return pa.table({"c0": r.reshape(-1), "c1": c.reshape(-1), "v": vals})
+    q = grid(nq, d, rng.normal(size=nq * d) * 0.3)
+    k = grid(nk, d, rng.normal(size=nk * d) * 0.3)
+    v = grid(nk, 1, rng.normal(size=nk))
+    w = ir.Of(ir.Expression.lit(1.0) / (ir.Expression.lit(1.0) + ir.Expression.col(0, 0) * ir.Expression.col(0, 0)))
+    p = attention_pipeline([], False, weight=w)
+    got, want = both(p, q, k, v)
+    compare(got, want, 1e-9 * np.max(np.abs(v.column("v").to_numpy())))
