@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 256 MiB device copy first (a known-size kernel for counter scaling)")
     ap.add_argument("--line", choices=["dense_fold", "ranking", "batched_folds", "large_fold", "smoother",
-                                       "regridding", "chain"],
+                                       "regridding", "chain", "sparse_product"],
                     help="print one secondary line alone (for profiling one workload) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -76,7 +76,8 @@ def main():
         one = {"dense_fold": lambda: dense_fold_line(dev, not args.no_cpu), "ranking": lambda: ranking_line(dev, not args.no_cpu),
                "batched_folds": lambda: batched_folds_line(dev), "large_fold": lambda: large_fold_line(dev),
                "smoother": smoother_line,
-               "regridding": regrid_line, "chain": lambda: chain_line(not args.no_cpu)}[args.line]
+               "regridding": regrid_line, "chain": lambda: chain_line(not args.no_cpu),
+               "sparse_product": lambda: sparse_product_line(not args.no_cpu)}[args.line]
         print(json.dumps({args.line: one()}), flush=True)
         return
     q64, k64, v64 = data.attention_tables(H, T, D, seed=3)
@@ -175,7 +176,8 @@ def main():
         secondary = {"dense_fold": dense_fold_line(dev, not args.no_cpu), "ranking": ranking_line(dev, not args.no_cpu),
                      "batched_folds": batched_folds_line(dev), "large_fold": large_fold_line(dev),
                      "smoother": smoother_line(),
-                     "regridding": regrid_line(), "chain": chain_line(not args.no_cpu)}
+                     "regridding": regrid_line(), "chain": chain_line(not args.no_cpu),
+                     "sparse_product": sparse_product_line(not args.no_cpu)}
 
     line = {
         "metric": "fold GFLOP/s + attention fwd+bwd ms: mixed vs plain FP32, 1 MI355X",
@@ -457,6 +459,68 @@ def regrid_line():
     return {"workload": "regridding 100k dst x 16 src, 365 steps (a tenth of 1M dst)", "algorithm": out.algorithm,
             "fma": fma, "rows_out": rows, "ms_arrow_to_arrow": round(ms, 2),
             "gflop_per_s": round(2 * fma / (ms * 1e-3) / 1e9, 2)}
+
+
+def sparse_product_line(with_cpu: bool):
+    """Config 2's sorting-path product (radix ranking, counting sorts,
+    Gustavson's product) through ef_fold, Arrow in and out: a(i, j, v) and
+    b(j, k, v) with 100k rows each, j = UInt64 hashes of Zipf(1.1) ids over
+    1M ids (as config 2), i and k = UInt64 hashes of 4096 ids each (config 2
+    has 64k: at this row count its groups would be about all 1.5e8 joined
+    pairs, 3.6 GB of Arrow), v ~ N(0, 1) as Float32 (accept_float32; the
+    FP64 precision widens them exactly).  FLOP = 2 per joined pair.  With
+    the CPU leg, the oracle's restatement on one core, whose groups and sums
+    the device's must equal bit for bit."""
+    import pyarrow as pa
+    import pyarrow.compute as pc
+    from einfold_amd import data, ir, kernels
+    rows = 100_000
+    a = pa.table({"i": pa.array(data.hash64(data.zipf_ids(8, 1, rows, 4096, 0.0), 0x11), pa.uint64()),
+                  "j": pa.array(data.hash64(data.zipf_ids(8, 2, rows, 1 << 20, 1.1), 0x6A), pa.uint64()),
+                  "v": pa.array(data.normal(8, 3, rows).astype(np.float32))})
+    b = pa.table({"j": pa.array(data.hash64(data.zipf_ids(8, 4, rows, 1 << 20, 1.1), 0x6A), pa.uint64()),
+                  "k": pa.array(data.hash64(data.zipf_ids(8, 5, rows, 4096, 0.0), 0x22), pa.uint64()),
+                  "v": pa.array(data.normal(8, 6, rows).astype(np.float32))})
+    EQ = ir.KeyEquality.EQUAL
+    f = ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2), ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)],
+                [ir.Output(0), ir.Output(2)])
+    opts = kernels.Options(accept_float32=True)
+    out = kernels.fold(f, a.to_batches(max_chunksize=65536), b.to_batches(max_chunksize=65536), options=opts)
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = kernels.fold(f, a.to_batches(max_chunksize=65536), b.to_batches(max_chunksize=65536), options=opts)
+        walls.append(time.perf_counter() - t0)
+    ja, jb = a.column(1).to_numpy(), b.column(0).to_numpy()
+    ua, ca = np.unique(ja, return_counts=True)
+    ub, cb = np.unique(jb, return_counts=True)
+    common, ia, ib = np.intersect1d(ua, ub, return_indices=True)
+    pairs = int((ca[ia].astype(np.int64) * cb[ib]).sum())
+    groups = sum(x.num_rows for x in out.batches)
+    ms = float(np.median(walls)) * 1e3
+    line = {"workload": "sparse product, 100k x 100k rows, UInt64 hashed keys, j Zipf(1.1) over 1M ids, i and k over "
+                        "4096 ids, Float32 factors (sorting path, Arrow to Arrow)",
+            "algorithm": out.algorithm, "joined_pairs": pairs, "hottest_j_rows": [int(ca.max()), int(cb.max())],
+            "groups": groups, "arrow_to_arrow_ms": round(ms, 2),
+            "gflop_per_s": round(2 * pairs / (ms * 1e-3) / 1e9, 2)}
+    if with_cpu:
+        from oracle import reference as R
+        wa = pa.table([pc.cast(c, pa.float64()) if c.type == pa.float32() else c for c in a.columns],
+                      names=a.column_names)
+        wb = pa.table([pc.cast(c, pa.float64()) if c.type == pa.float32() else c for c in b.columns],
+                      names=b.column_names)
+        t0 = time.perf_counter()
+        _, want = R.fold(f, wa.to_batches(), wb.to_batches(), None, R.schema(f, [wa.schema, wb.schema]))
+        line["cpu_port_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        line["cpu_port_cores"] = 1
+        tg, tw = pa.Table.from_batches(out.batches), pa.Table.from_batches(want)
+        same = tg.num_rows == tw.num_rows and all(
+            np.array_equal(tg.column(c).to_numpy().view(np.uint64), tw.column(c).to_numpy().view(np.uint64))
+            for c in range(3))
+        if not same:
+            raise SystemExit("sparse product: device groups or sums differ from the oracle's")
+        line["bit_exact_vs_oracle"] = True
+    return line
 
 
 def ranking_line(dev, with_cpu: bool = False):

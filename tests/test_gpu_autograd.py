@@ -14,7 +14,7 @@ import pyarrow as pa
 import pytest
 import torch
 
-from einfold_amd import autograd, data, ir, kernels
+from einfold_amd import autograd, data, device, ir, kernels
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +125,23 @@ def test_gradients_equal_the_left_joined_gradient_folds(cuda, wrt):
     got = tab.column(3).to_numpy().reshape(h, t, d)
     err = np.max(np.abs(got - grad)) / np.max(np.abs(grad))
     assert err < 1e-11, (wrt, err)
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 1e-5), ("fp32", 2e-6)])
+def test_mixed_backward_at_the_gradient_workload_against_fp64(cuda, precision, tol):
+    """The gradient workload at full size (4 heads, T = 2048, d = 64,
+    causal): out, dq, dk and dv of the mixed (BF16x3) and plain FP32 forms
+    against the FP64 form on the same device, each within the 1e-5 relative
+    bar (norm of the difference over the norm of the FP64 result)."""
+    print(f"full-size backward {precision}", flush=True)
+    h, t, d, scale = 4, 2048, 64, 0.125
+    q, k, v, g = _tables(h, t, d, 11)
+    t64 = [torch.from_numpy(np.ascontiguousarray(x)).to(cuda) for x in (q, k, v, g)]
+    t32 = [x.float() for x in t64]
+    o64, m64, l64 = device.attention_fwd(*t64[:3], scale, causal=True, precision="fp64")
+    g64 = device.attention_bwd(*t64[:3], o64, t64[3], m64, l64, scale, causal=True, precision="fp64")
+    o32, m32, l32 = device.attention_fwd(*t32[:3], scale, causal=True, precision=precision)
+    g32 = device.attention_bwd(*t32[:3], o32, t32[3], m32, l32, scale, causal=True, precision=precision)
+    for name, got, want in zip(("out", "dq", "dk", "dv"), (o32,) + tuple(g32), (o64,) + tuple(g64)):
+        err = float(torch.linalg.vector_norm(got.double() - want) / torch.linalg.vector_norm(want))
+        assert err < tol, (name, err)
