@@ -3,6 +3,8 @@
 
 #include <cstdlib>
 #include <cstring>
+
+#include <hip/hip_runtime.h>
 #include <new>
 
 #include "common.h"
@@ -111,14 +113,33 @@ bool read_relation(const ef_relation *in, Relation &out) {
     return true;
 }
 
+// Output buffers from 1 MiB up are pinned (hipHostMalloc): the results'
+// device-to-host copies then run as direct DMA into them, where a fresh
+// pageable buffer would be staged in chunks and take a page fault on first
+// touch of each page (about 13x slower for hundreds of MB).  Small ones
+// stay in pageable memory, where pinning would cost more than it saves.
+constexpr size_t PIN_FROM = (size_t)1 << 20;
+
 HostBuffer::HostBuffer(size_t n) : bytes(n) {
     const size_t padded = ((n + 63) / 64) * 64 + 64;
-    data = static_cast<uint8_t *>(std::aligned_alloc(64, padded));
+    if (padded >= PIN_FROM) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, padded, hipHostMallocDefault) == hipSuccess && p) {
+            data = static_cast<uint8_t *>(p);
+            pinned = true;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    if (!data) data = static_cast<uint8_t *>(std::aligned_alloc(64, padded));
     if (!data) throw std::bad_alloc();
     std::memset(data + (padded - 64), 0, 64);
 }
 
-HostBuffer::~HostBuffer() { std::free(data); }
+HostBuffer::~HostBuffer() {
+    if (pinned) (void)hipHostFree(data);
+    else std::free(data);
+}
 
 namespace {
 

@@ -52,10 +52,11 @@ struct Relation {
 // Reads an ef_relation; false (with the error set) if malformed.
 bool read_relation(const ef_relation *in, Relation &out);
 
-// Host buffer aligned to 64 bytes, as Arrow recommends.
+// Host buffer aligned to 64 bytes, as Arrow recommends; pinned when large.
 struct HostBuffer {
     uint8_t *data = nullptr;
     size_t bytes = 0;
+    bool pinned = false;
     explicit HostBuffer(size_t n);
     ~HostBuffer();
     HostBuffer(const HostBuffer &) = delete;
