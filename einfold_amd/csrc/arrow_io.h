@@ -71,6 +71,10 @@ struct OutColumn {
     int64_t null_count = 0;
     std::shared_ptr<HostBuffer> values;
     std::shared_ptr<HostBuffer> validity;   // null: no NULLs
+    // Set while the column's NULL count is still being copied from the
+    // device: after the stream is synchronized, settle() reads it and drops
+    // a bitmap that marks no NULL.
+    std::shared_ptr<unsigned long long> pending_nulls;
 };
 
 struct OutField {

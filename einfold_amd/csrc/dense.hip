@@ -99,6 +99,72 @@ __global__ __launch_bounds__(256) void dense_accumulate_kernel(const double *__r
     }
 }
 
+// Outputs too small to fill the chip with the tiles above (256^3 makes 64
+// workgroups of 32 x 32): one cell per thread, 16 x 16 cells per
+// workgroup, z in chunks of 64 staged through LDS, the next chunk loaded
+// into registers while this one is summed.  Each cell's chain is the same
+// z-ordered sequence of roundings, so the same bits.
+template <bool PRED>
+__global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__restrict__ a,
+                                                             const uint32_t *__restrict__ held,
+                                                             const double *__restrict__ b, int64_t nx, int64_t nz,
+                                                             int64_t ny, int64_t per_batch, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    constexpr int T = 16, ZT = 64, PER = T * ZT / 256;   // 4 values of a and of b per thread per chunk
+    __shared__ double as[T][ZT + 1];
+    __shared__ uint8_t fs[T][ZT + 1];
+    __shared__ double bs[ZT][T];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int64_t w = blockIdx.z;
+    const int64_t x0 = w * per_batch + (int64_t)blockIdx.x * T;
+    const int64_t xend = w * per_batch + per_batch < nx ? w * per_batch + per_batch : nx;
+    const int64_t y0 = (int64_t)blockIdx.y * T;
+    const double *bw = b + w * nz * ny;
+    double ra[PER], rb[PER];
+    uint8_t rf[PER];
+    auto fetch = [&](int64_t z0) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + 256 * k;
+            const int r = i / ZT, zz = i % ZT;
+            const int64_t x = x0 + r, z = z0 + zz;
+            const bool in = x < xend && z < nz;
+            // Unconditional loads from a clamped cell, then 0 outside.
+            const int64_t xc = in ? x : x0, zcl = in ? z : 0;
+            const double av = a[xc * nz + zcl];
+            const bool v = in && held[xc * nz + zcl] == VALUED;
+            ra[k] = v ? av : 0.0;
+            rf[k] = v ? 1 : 0;
+            const int zb = i / T, cc = i % T;
+            const int64_t zb_g = z0 + zb, y = y0 + cc;
+            const bool inb = zb_g < nz && y < ny;
+            const double bv = bw[(inb ? zb_g : 0) * ny + (inb ? y : 0)];
+            rb[k] = inb ? bv : 0.0;
+        }
+    };
+    double acc = 0.0;
+    if (nz > 0) fetch(0);
+    for (int64_t z0 = 0; z0 < nz; z0 += ZT) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + 256 * k;
+            as[i / ZT][i % ZT] = ra[k];
+            fs[i / ZT][i % ZT] = rf[k];
+            bs[i / T][i % T] = rb[k];
+        }
+        __syncthreads();
+        if (z0 + ZT < nz) fetch(z0 + ZT);
+        const int zn = nz - z0 < ZT ? (int)(nz - z0) : ZT;
+        for (int zz = 0; zz < zn; zz++) {
+            const double t = acc + as[ty][zz] * bs[zz][tx];
+            acc = (!PRED || fs[ty][zz]) ? t : acc;
+        }
+        __syncthreads();
+    }
+    const int64_t x = x0 + ty, y = y0 + tx;
+    if (x < xend && y < ny) out[x * ny + y] = acc;
+}
+
 // Outputs of fewer than 4096 cells (dense.rs:752-768): z in blocks of
 // 4096, each block's partial summed from +0.0, the partials added to +0.0
 // in block order.  One thread per cell.
@@ -217,8 +283,25 @@ ef_status launch_dense_accumulate_f64(const double *a, const uint32_t *held, con
         hipLaunchKernelGGL(dense_accumulate_small, dim3(blocks_for(nx * ny, 256)), dim3(256), 0, s,
                            a, held, b, (int64_t)nx, (int64_t)nz, (int64_t)ny, (int64_t)per_batch, out);
     } else {
-        // 64 x 64 tiles when they fill the chip, else 32 x 32.
+        // 64 x 64 tiles when they fill the chip, else 32 x 32, else (fewer
+        // than 512 of those) one cell per thread.
         const bool big = ((per_batch + 63) / 64) * ((ny + 63) / 64) * (size_t)nw >= 1024;
+        const bool thin = ((per_batch + 31) / 32) * ((ny + 31) / 32) * (size_t)nw < 512;
+        if (thin) {
+            if (nw > 65535 || (ny + 15) / 16 > 65535 || (per_batch + 15) / 16 > 0x7fffffffULL) {
+                set_error("too many batches or columns for one launch");
+                return EF_INVALID;
+            }
+            dim3 tgrid((unsigned)((per_batch + 15) / 16), (unsigned)((ny + 15) / 16), (unsigned)nw);
+            if (predicated)
+                hipLaunchKernelGGL(dense_accumulate_thin<true>, tgrid, dim3(256), 0, s, a, held, b, (int64_t)nx,
+                                   (int64_t)nz, (int64_t)ny, (int64_t)per_batch, out);
+            else
+                hipLaunchKernelGGL(dense_accumulate_thin<false>, tgrid, dim3(256), 0, s, a, held, b, (int64_t)nx,
+                                   (int64_t)nz, (int64_t)ny, (int64_t)per_batch, out);
+            EF_HIP(hipGetLastError());
+            return EF_OK;
+        }
         const int tm = big ? 64 : 32, tn = big ? 64 : 32;
         const int64_t x_tiles = ((int64_t)per_batch + tm - 1) / tm;
         if (nw > 65535 || (ny + tn - 1) / tn > 65535) {

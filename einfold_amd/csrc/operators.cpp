@@ -300,28 +300,33 @@ ef_status fetch_column(Call &c, Type t, const void *values, const uint8_t *valid
     const int w = width_of(t);
     out.values = std::make_shared<HostBuffer>((size_t)n * w);
     EF_TRY(c.download<uint8_t>(out.values->data, static_cast<const uint8_t *>(values), (size_t)n * w));
-    unsigned long long nulls = 0;
-    std::shared_ptr<HostBuffer> bits;
     if (valid && n) {
         uint8_t *dbits;
         unsigned long long *dnulls;
         EF_TRY(c.alloc(dbits, (size_t)(n + 7) / 8));
         EF_TRY(c.zeros(dnulls, 1));
         EF_TRY(pack_bits(n, valid, dbits, dnulls, c.stream));
-        bits = std::make_shared<HostBuffer>((size_t)(n + 7) / 8);
-        EF_TRY(c.download<uint8_t>(bits->data, dbits, (size_t)(n + 7) / 8));
-        EF_TRY(c.download<unsigned long long>(&nulls, dnulls, 1));
-    }
-    EF_TRY(c.sync());
-    if (nulls) {
-        out.validity = bits;
-        out.null_count = (int64_t)nulls;
+        out.validity = std::make_shared<HostBuffer>((size_t)(n + 7) / 8);
+        out.pending_nulls = std::make_shared<unsigned long long>(0);
+        EF_TRY(c.download<uint8_t>(out.validity->data, dbits, (size_t)(n + 7) / 8));
+        EF_TRY(c.download<unsigned long long>(out.pending_nulls.get(), dnulls, 1));
     }
     return EF_OK;
 }
 
+void settle(std::vector<std::vector<OutColumn>> &batches) {
+    for (auto &cols : batches)
+        for (auto &col : cols) {
+            if (!col.pending_nulls) continue;
+            col.null_count = (int64_t)*col.pending_nulls;
+            if (col.null_count == 0) col.validity.reset();
+            col.pending_nulls.reset();
+        }
+}
+
 void put_result(std::vector<std::vector<OutColumn>> batches, const std::vector<OutField> &fields, uint32_t algorithm,
                 ef_result *out) {
+    settle(batches);
     std::memset(out, 0, sizeof *out);
     export_schema(fields, &out->schema);
     out->n_batches = batches.size();
@@ -1912,6 +1917,7 @@ ef_status fold_chain_impl(const FoldModel &m, const std::vector<const Relation *
     if (!fold_schema(first, {ops[x], ops[y]}, nullptr, accept32, first_fields)) return EF_INVALID;
     std::vector<std::vector<OutColumn>> t;
     EF_TRY(chain_either(first, *ops[x], *ops[y], first_fields, options, t));
+    settle(t);
     const Relation rt = chain_relation(t, first_fields);
     return chain_either(second, rt, *ops[rest], fields, options, out);
 }
@@ -1966,6 +1972,7 @@ ef_status ef_fold_dense(const ef_fold_desc *fold, const ef_relation *a, const ef
             (void)hipStreamSynchronize(c.stream);
             return st;
         }
+        EF_TRY(c.sync());
         put_result(std::move(batches), fields, EF_ALGORITHM_DENSE, out);
         return EF_OK;
     });
@@ -2023,17 +2030,16 @@ ef_status ef_fold_dense_many(size_t count, const ef_fold_desc *const *folds, con
                                     batch.density.data(), c.stream));
         for (size_t i = 0; i < count; i++) {
             One &o = ones[i];
-            if (status[i] != EF_OK) continue;
-            if (o.finish) {
-                const ef_status st = o.finish();
-                if (st != EF_OK) {
-                    (void)hipStreamSynchronize(c.stream);
-                    return st;
-                }
+            if (status[i] != EF_OK || !o.finish) continue;
+            const ef_status st = o.finish();
+            if (st != EF_OK) {
+                (void)hipStreamSynchronize(c.stream);
+                return st;
             }
-            put_result(std::move(o.batches), o.fields, EF_ALGORITHM_DENSE, &out[i]);
         }
         EF_HIP(hipStreamSynchronize(c.stream));
+        for (size_t i = 0; i < count; i++)
+            if (status[i] == EF_OK) put_result(std::move(ones[i].batches), ones[i].fields, EF_ALGORITHM_DENSE, &out[i]);
         return EF_OK;
     });
 }
@@ -2086,6 +2092,7 @@ ef_status ef_fold(const ef_fold_desc *fold, const ef_relation *a, const ef_relat
             (void)hipStreamSynchronize(c.stream);
             return st;
         }
+        EF_TRY(c.sync());
         put_result(std::move(batches), fields, EF_ALGORITHM_DISCRIMINATE, out);
         return EF_OK;
     });
@@ -2159,6 +2166,7 @@ ef_status ef_pipeline(const ef_pipeline_desc *pipeline, const ef_relation *q, co
             (void)hipStreamSynchronize(c.stream);
             return st;
         }
+        EF_TRY(c.sync());
         put_result(std::move(batches), fields, EF_ALGORITHM_FUSED, out);
         out->tiles_visited = visited;
         out->tiles = tiles;

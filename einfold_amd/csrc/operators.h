@@ -101,11 +101,17 @@ ef_status upload_raw(Call &c, const Relation &r, int column, void *dst);
 OutColumn copy_column(const ColumnView &v, int64_t rows, bool nullable);
 
 // Builds an output column from device values (width bytes each) and an
-// optional device byte-per-row validity; synchronizes.
+// optional device byte-per-row validity.  The copies are only enqueued on
+// the call's stream: the column is complete once the stream has been
+// synchronized and settle() has run.
 ef_status fetch_column(Call &c, Type t, const void *values, const uint8_t *valid, int64_t n, bool nullable,
                        OutColumn &out);
+// Completes the columns fetch_column built (their NULL counts); the
+// stream that copied them must have been synchronized.
+void settle(std::vector<std::vector<OutColumn>> &batches);
 
-// Moves batches and a schema into an ef_result.
+// Moves batches and a schema into an ef_result (settling them first: their
+// copies must have finished).
 void put_result(std::vector<std::vector<OutColumn>> batches, const std::vector<OutField> &fields, uint32_t algorithm,
                 ef_result *out);
 
