@@ -10,6 +10,8 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <map>
+#include <mutex>
 
 #include "discriminate.h"
 #include "pipeline.h"
@@ -198,6 +200,57 @@ bool fold_schema(const FoldModel &m, const std::vector<const Relation *> &operan
         fields.push_back(OutField{Type::Float64, !m.has_missing});
     }
     return true;
+}
+
+namespace {
+std::mutex chunk_mu;
+std::multimap<std::pair<int, size_t>, void *> chunk_cache;   // (device, bytes) -> chunk
+size_t chunk_cached = 0;
+constexpr size_t CHUNK_CAP = (size_t)2 << 30;                 // bytes kept between calls, per process
+}  // namespace
+
+bool chunk_take(size_t bytes, DeviceBuffer &out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    std::lock_guard<std::mutex> lock(chunk_mu);
+    auto it = chunk_cache.lower_bound({dev, bytes});
+    if (it == chunk_cache.end() || it->first.first != dev) return false;
+    out.ptr = it->second;
+    out.bytes = it->first.second;
+    chunk_cached -= out.bytes;
+    chunk_cache.erase(it);
+    return true;
+}
+
+void chunk_give(DeviceBuffer &&chunk) {
+    if (!chunk.ptr) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        std::lock_guard<std::mutex> lock(chunk_mu);
+        if (chunk_cached + chunk.bytes <= CHUNK_CAP) {
+            chunk_cache.emplace(std::make_pair(dev, chunk.bytes), chunk.ptr);
+            chunk_cached += chunk.bytes;
+            chunk.ptr = nullptr;
+            chunk.bytes = 0;
+            return;
+        }
+    }
+    chunk.alloc(0);   // frees it
+}
+
+void chunk_drop() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lock(chunk_mu);
+    for (auto it = chunk_cache.begin(); it != chunk_cache.end();) {
+        if (it->first.first != dev) {
+            ++it;
+            continue;
+        }
+        (void)hipFree(it->second);
+        chunk_cached -= it->first.second;
+        it = chunk_cache.erase(it);
+    }
 }
 
 ef_status Call::reserve(double bytes, const char *what) {

@@ -54,21 +54,71 @@ bool parse_fold(const ef_fold_desc *f, FoldModel &m);
 bool fold_schema(const FoldModel &m, const std::vector<const Relation *> &operands, const Relation *domain,
                  bool accept_float32, std::vector<OutField> &fields);
 
-// One call's options, stream, budget and device memory.
+// Device memory chunks kept between calls (per device, up to a cap), so
+// that calls in steady state allocate nothing: chunk_take gives the
+// smallest cached chunk of at least `bytes` (false: none), chunk_give keeps
+// or frees a chunk the device no longer uses, chunk_drop frees them all
+// (before retrying an allocation that ran short).
+bool chunk_take(size_t bytes, DeviceBuffer &out);
+void chunk_give(DeviceBuffer &&chunk);
+void chunk_drop();
+
+// One call's options, stream, budget and device memory.  The call's
+// device buffers are carved from a few large chunks (at least 4 MiB,
+// doubling), taken from the chunk cache where it has one: the dozens of
+// small buffers a fold needs cost no hipMalloc once a call of that size
+// has run.  When the call ends its stream is synchronized and the chunks go
+// back to the cache.
 struct Call {
     const ef_options *options = nullptr;
     hipStream_t stream = nullptr;
-    std::deque<DeviceBuffer> buffers;
+    std::deque<DeviceBuffer> buffers;   // the chunks
+    size_t used = 0;                    // bytes carved from the last chunk
 
     explicit Call(const ef_options *o) : options(o), stream(as_stream(o ? o->stream : nullptr)) {}
+    ~Call() {
+        if (buffers.empty()) return;
+        (void)hipStreamSynchronize(stream);
+        for (auto &b : buffers) chunk_give(std::move(b));
+    }
+    Call(const Call &) = delete;
+    Call &operator=(const Call &) = delete;
 
     // options.rs:95-103: asks the budget, EF_OUT_OF_MEMORY if it says no.
     ef_status reserve(double bytes, const char *what);
 
+    // n bytes of device memory, 256-byte aligned, alive until the call ends.
+    ef_status carve(size_t n, void *&p) {
+        n = (n + 255) & ~(size_t)255;
+        if (buffers.empty() || used + n > buffers.back().bytes) {
+            const size_t last = buffers.empty() ? 0 : buffers.back().bytes;
+            size_t chunk = (size_t)4 << 20;
+            if (chunk < 2 * last) chunk = 2 * last;
+            if (chunk < n) chunk = n;
+            buffers.emplace_back();
+            if (!chunk_take(n, buffers.back())) {
+                ef_status st = buffers.back().alloc(chunk);
+                if (st != EF_OK) {
+                    chunk_drop();   // the cache's chunks back to the device, then just what is asked
+                    st = buffers.back().alloc(n);
+                }
+                if (st != EF_OK) {
+                    buffers.pop_back();
+                    p = nullptr;
+                    return st;
+                }
+            }
+            used = 0;
+        }
+        p = static_cast<char *>(buffers.back().ptr) + used;
+        used += n;
+        return EF_OK;
+    }
+
     template <class T> ef_status alloc(T *&p, size_t n) {
-        buffers.emplace_back();
-        ef_status st = buffers.back().alloc(n * sizeof(T) + 16);
-        p = buffers.back().as<T>();
+        void *v = nullptr;
+        ef_status st = carve(n * sizeof(T) + 16, v);
+        p = static_cast<T *>(v);
         return st;
     }
     template <class T> ef_status zeros(T *&p, size_t n) {
