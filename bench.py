@@ -524,7 +524,8 @@ def sparse_product_line(with_cpu: bool):
 
 
 def ranking_line(dev, with_cpu: bool = False):
-    """Radix ranking of the sorting path's join key: j over both operands,
+    """Ranking of the sorting path's join key (hash dedup, then the distinct
+    words ordered by bucket): j over both operands,
     8M UInt64 hashes of Zipf(1.1) ids over 1M ids.  Algorithmic traffic 12 B
     per key (read the 8-byte word, write the 4-byte rank).  With the CPU
     leg, the oracle's C restatement of discriminate.rs's ranking on one
@@ -548,7 +549,7 @@ def ranking_line(dev, with_cpu: bool = False):
     run = lambda: check(lib().ef_rank_u64(P(words), n, P(rank), P(first), ctypes.byref(k), P(scratch), stream))
     ms = _events(run, 10)
     gbs = 12.0 * n / (ms * 1e-3) / 1e9
-    line = {"workload": "radix ranking of 8M UInt64 join keys (Zipf 1.1 over 1M ids)", "keys": n,
+    line = {"workload": "ranking of 8M UInt64 join keys (Zipf 1.1 over 1M ids): hash dedup, distinct words ordered by bucket", "keys": n,
             "distinct": int(k.value), "ms": round(ms, 4), "gb_per_s": round(gbs, 1),
             "frac_of_hbm_peak": round(gbs / 8000.0, 4)}
     if with_cpu:

@@ -1,5 +1,15 @@
-// Radix ranking on gfx950 (crates/einfold-kernels/src/discriminate.rs:
-// 103-156, 280-292): a stable least-significant-byte-first sort of
+// Ranking and sorting on gfx950 (crates/einfold-kernels/src/
+// discriminate.rs:103-156, 280-292): each row's rank among the distinct
+// words and each rank's first row, and a stable sort of (word, row) pairs.
+//
+// Ranking from 65536 rows first deduplicates the words through a hash
+// table and orders only the distinct ones (the hashing path, below the
+// radix kernels); where that cannot finish (more distinct words than a
+// quarter of the rows, or words crowded into a narrow part of their range)
+// it ranks by the radix sort.  Both give the same ranks and first rows:
+// integer results that depend on no order of execution.
+//
+// The radix sort: a stable least-significant-byte-first sort of
 // (word, row) pairs that skips the bytes no word varies in, then each
 // row's rank among the distinct words and each rank's first row.
 //
@@ -552,6 +562,307 @@ __global__ __launch_bounds__(256) void iota_kernel(uint32_t *out, int64_t n) {
     if (i < n) out[i] = (uint32_t)i;
 }
 
+// ---------------------------------------------------------------------
+// Ranking by hashing, for inputs whose distinct words are few next to the
+// rows (join and grouping keys usually are): the words are deduplicated
+// first and only the distinct ones are ordered.
+//  1. Each workgroup takes 1024 consecutive rows and deduplicates them in
+//     an LDS table (lowest row of each word by LDS atomics), then enters
+//     each of its distinct words once into a device table of `cap` slots
+//     (open addressing, linear probing, compare-and-swap from EMPTY), takes
+//     its lowest row into the slot's first-row word (atomicMin, skipped
+//     when the word's first row is already lower: the device-wide atomics
+//     on a frequent word's slot would otherwise queue one per workgroup)
+//     and writes every row's slot.  Its smallest and largest word go to an
+//     array of their own, reduced by one small launch.
+//  2. The table's occupied slots are counted into `nb` buckets of equal
+//     width between the smallest and the largest word, the counts scanned,
+//     and the words scattered by bucket; a word's rank is its bucket's
+//     start plus the words of its bucket below it (a bucket holds a few
+//     words when the words spread over their range; one of more than
+//     BIG_BUCKET words stops this path).
+//  3. Each row's rank is its slot's.
+// Atomics decide only where a word is stored, never a result: ranks (the
+// order of distinct words) and first rows (minima) are the same whatever
+// the interleaving.  When a probe runs past HPROBES slots (more distinct
+// words than the table takes) or a bucket is too large, the call ranks
+// again by the radix sort.  The word EMPTY, which marks free slots, is
+// kept apart in slot `cap` (present when its first row is set).
+// ---------------------------------------------------------------------
+constexpr int HTHREADS = 256;
+constexpr int HROWS = 1024;        // rows per workgroup of the insert pass
+constexpr int HSLOTS = 2048;       // its LDS table (twice its rows: never full)
+constexpr int HPROBES = 128;       // longest probe into the device table
+constexpr int BIG_BUCKET = 64;     // largest bucket ranked in place
+constexpr unsigned long long EMPTY = ~0ull;
+constexpr int64_t HASH_FROM = 1 << 16;   // rows from which ranking hashes first
+
+struct HashState {
+    unsigned long long kmin, kmax;   // smallest and largest word
+    unsigned flags;                  // nonzero: this path cannot finish
+};
+
+struct HashArgs {
+    const uint64_t *words;
+    int64_t n;
+    unsigned long long *key;   // [cap]
+    uint32_t *first;           // [cap + 2]: lowest row per slot; slot cap is EMPTY's
+    uint32_t *slot;            // [n]
+    uint32_t *bcount;          // [nb + 1]: bucket counts, scanned in place to starts
+    uint32_t *bpos;            // [cap + 1]: each word's place in its bucket
+    uint64_t *bkeys;           // [cap + 1]: the words by bucket
+    uint32_t *rank_of;         // [cap + 2]
+    unsigned long long *wg_range;   // [2 * workgroups of the insert pass]: smallest, largest word
+    HashState *st;
+    uint32_t cap, nb;
+    int nb_log;
+    uint32_t *rank, *first_out;
+};
+
+// MurmurHash3's 64-bit finalizer (Appleby): spreads clustered words over
+// the table.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void hash_clear_kernel(HashArgs a) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)a.cap + 2; i += stride) {
+        if (i < a.cap) a.key[i] = EMPTY;
+        a.first[i] = ~0u;
+        if (i <= a.nb) a.bcount[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.st->kmin = EMPTY;
+        a.st->kmax = 0;
+        a.st->flags = 0;
+    }
+}
+
+__global__ __launch_bounds__(HTHREADS) void hash_insert_kernel(HashArgs a) {
+    __shared__ unsigned long long lkey[HSLOTS];
+    // Per local slot (HSLOTS: the word EMPTY's): its lowest row, replaced
+    // by its device slot once the word is entered.
+    __shared__ uint32_t lmin[HSLOTS + 1];
+    __shared__ uint32_t act[HROWS + 1];     // the occupied local slots
+    __shared__ unsigned long long wlo[HTHREADS / 64], whi[HTHREADS / 64];
+    __shared__ unsigned nact, wg_fail;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * HROWS;
+    for (int i = tid; i < HSLOTS; i += HTHREADS) {
+        lkey[i] = EMPTY;
+        lmin[i] = ~0u;
+    }
+    if (tid == 0) {
+        lmin[HSLOTS] = ~0u;
+        nact = 0;
+        wg_fail = 0;
+    }
+    __syncthreads();
+    constexpr int PER = HROWS / HTHREADS;
+    int ls[PER];
+    unsigned long long words[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {   // every load in flight at once (row clamped)
+        const int64_t row = r0 + k * HTHREADS + tid;
+        words[k] = a.words[row < a.n ? row : a.n - 1];
+    }
+    unsigned long long lo = EMPTY, hi = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int64_t row = r0 + k * HTHREADS + tid;
+        ls[k] = -1;
+        if (row >= a.n) continue;
+        const unsigned long long w = words[k];
+        lo = w < lo ? w : lo;
+        hi = w > hi ? w : hi;
+        int h = HSLOTS;
+        if (w != EMPTY) {
+            h = (int)(mix64(w) & (HSLOTS - 1));
+            for (;;) {   // ends: the table has twice the workgroup's rows
+                unsigned long long cur = __hip_atomic_load(&lkey[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cur == EMPTY) {
+                    cur = atomicCAS(&lkey[h], EMPTY, w);
+                    if (cur == EMPTY) break;
+                }
+                if (cur == w) break;
+                h = (h + 1) & (HSLOTS - 1);
+            }
+        }
+        ls[k] = h;
+        if (__hip_atomic_load(&lmin[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > (uint32_t)row)
+            atomicMin(&lmin[h], (uint32_t)row);
+    }
+    // The workgroup's smallest and largest word (lanes without rows hold
+    // EMPTY and 0, which change neither).
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if (lane == 0) {
+        wlo[wave] = lo;
+        whi[wave] = hi;
+    }
+    __syncthreads();
+    // The occupied local slots, packed (in any order).
+    for (int base = 0; base <= HSLOTS; base += HTHREADS) {
+        const int i = base + tid;
+        const bool occupied = i <= HSLOTS && lmin[i] != ~0u;
+        const unsigned long long mask = __ballot(occupied);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned at = 0;
+            if (lane == leader) at = atomicAdd(&nact, (unsigned)__popcll(mask));
+            at = __shfl(at, leader) + (unsigned)__popcll(mask & ((1ull << lane) - 1));
+            if (occupied) act[at] = (uint32_t)i;
+        }
+    }
+    __syncthreads();
+    // Each distinct word of the workgroup into the device table.  A plain
+    // load of a slot may return an older value from a cache: EMPTY (the
+    // compare-and-swap then reads the slot as it is) or the word the slot
+    // holds now (a slot changes once).  A first-row word only decreases,
+    // so a value read earlier is never below the present one: when it is
+    // not above this workgroup's row, neither is the present one.
+    const unsigned na = nact;
+    for (unsigned j = tid; j < na; j += HTHREADS) {
+        const uint32_t i = act[j];
+        const uint32_t mn = lmin[i];
+        uint32_t g = a.cap + 1;   // no slot: the path fails, and slot cap + 1 is never read for a result
+        if (i == HSLOTS) {
+            g = a.cap;
+        } else {
+            const unsigned long long w = lkey[i];
+            uint32_t h = (uint32_t)(mix64(w) & (a.cap - 1));
+            for (int p = 0; p < HPROBES; p++) {
+                unsigned long long cur = a.key[h];
+                if (cur == EMPTY) cur = atomicCAS(&a.key[h], EMPTY, w);
+                if (cur == EMPTY || cur == w) {
+                    g = h;
+                    break;
+                }
+                h = (h + 1) & (a.cap - 1);
+            }
+            if (g == a.cap + 1) wg_fail = 1;
+        }
+        if (g <= a.cap && __hip_atomic_load(&a.first[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > mn)
+            atomicMin(&a.first[g], mn);
+        lmin[i] = g;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (wg_fail) atomicOr(&a.st->flags, 1u);
+        unsigned long long l = wlo[0], h = whi[0];
+        for (int v = 1; v < HTHREADS / 64; v++) {
+            l = wlo[v] < l ? wlo[v] : l;
+            h = whi[v] > h ? whi[v] : h;
+        }
+        a.wg_range[2 * blockIdx.x] = l;
+        a.wg_range[2 * blockIdx.x + 1] = h;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++)
+        if (ls[k] >= 0) a.slot[r0 + k * HTHREADS + tid] = lmin[ls[k]];
+}
+
+// The smallest and largest word over the insert pass's workgroups (one
+// workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void hash_range_kernel(HashArgs a, int groups) {
+    __shared__ unsigned long long slo[16], shi[16];
+    unsigned long long lo = EMPTY, hi = 0;
+    for (int i = threadIdx.x; i < groups; i += 1024) {
+        const unsigned long long l = a.wg_range[2 * i], h = a.wg_range[2 * i + 1];
+        lo = l < lo ? l : lo;
+        hi = h > hi ? h : hi;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 1; v < 16; v++) {   // lo, hi: wave 0's
+            lo = slo[v] < lo ? slo[v] : lo;
+            hi = shi[v] > hi ? shi[v] : hi;
+        }
+        a.st->kmin = lo;
+        a.st->kmax = hi;
+    }
+}
+
+// Whether table slot g (0 .. cap) holds a word, and which.
+__device__ __forceinline__ bool slot_word(const HashArgs &a, int64_t g, uint64_t &w) {
+    if (g == a.cap) {
+        w = EMPTY;
+        return a.first[g] != ~0u;
+    }
+    w = a.key[g];
+    return w != EMPTY;
+}
+
+// The bucket of a word: nb buckets of equal width from kmin to kmax.
+__device__ __forceinline__ uint32_t bucket_of(uint64_t w, const HashState &st, int nb_log) {
+    const uint64_t range = st.kmax - st.kmin;
+    const int bits = range ? 64 - __clzll((long long)range) : 0;
+    const int shift = bits > nb_log ? bits - nb_log : 0;
+    return (uint32_t)((w - st.kmin) >> shift);
+}
+
+__global__ __launch_bounds__(256) void bucket_count_kernel(HashArgs a) {
+    const HashState st = *a.st;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= (int64_t)a.cap; g += stride) {
+        uint64_t w;
+        if (slot_word(a, g, w)) a.bpos[g] = atomicAdd(&a.bcount[bucket_of(w, st, a.nb_log)], 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void bucket_scatter_kernel(HashArgs a) {
+    const HashState st = *a.st;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= (int64_t)a.cap; g += stride) {
+        uint64_t w;
+        if (slot_word(a, g, w)) a.bkeys[a.bcount[bucket_of(w, st, a.nb_log)] + a.bpos[g]] = w;
+    }
+}
+
+__global__ __launch_bounds__(256) void bucket_rank_kernel(HashArgs a) {
+    const HashState st = *a.st;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= (int64_t)a.cap; g += stride) {
+        uint64_t w;
+        if (!slot_word(a, g, w)) continue;
+        const uint32_t b = bucket_of(w, st, a.nb_log);
+        const uint32_t lo = a.bcount[b], hi = a.bcount[b + 1];
+        if (hi - lo > (uint32_t)BIG_BUCKET) {
+            a.st->flags = 2u;   // any nonzero value stops the path
+            continue;
+        }
+        uint32_t below = 0;
+        for (uint32_t j = lo; j < hi; j++) below += a.bkeys[j] < w ? 1u : 0u;
+        a.rank_of[g] = lo + below;
+        a.first_out[lo + below] = a.first[g];
+    }
+}
+
+__global__ __launch_bounds__(256) void rank_rows_kernel(HashArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) a.rank[i] = a.rank_of[a.slot[i]];
+}
+
 size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t scan_temp_bytes(size_t n) {
@@ -586,10 +897,70 @@ ef_status scan_u32(const uint32_t *in, uint32_t *out, size_t n, void *tmp, hipSt
 
 size_t scan_scratch_bytes(size_t n) { return scan_temp_bytes(n); }
 
-size_t rank_scratch_bytes(size_t n) {
+namespace {
+
+size_t radix_bytes(size_t n) {
     const int64_t tiles = tiles_of(n);
     return round_up(n * 8) * 2 + round_up(n * 4) * 2 + round_up((size_t)tiles * 2048 * 4) + 2048 * 4 +
            2 * round_up((size_t)tiles * 257 * 4) + SMALL_WORDS * 4;
+}
+
+// Sizes of the hashing path for n rows: a device table of cap >= n / 2
+// slots, cap / 8 buckets.
+struct HashLayout {
+    uint32_t cap, nb;
+    int nb_log;
+};
+
+HashLayout hash_layout(size_t n) {
+    HashLayout L;
+    size_t cap = HSLOTS;
+    while (cap < (n + 1) / 2) cap <<= 1;
+    L.cap = (uint32_t)cap;
+    L.nb = L.cap / 8 > 256 ? L.cap / 8 : 256;
+    L.nb_log = 0;
+    while ((1u << L.nb_log) < L.nb) L.nb_log++;
+    return L;
+}
+
+// Carves the hashing path's arrays from `scratch` (a null scratch gives
+// the bytes needed, in `bytes`).
+HashArgs hash_carve(void *scratch, size_t n, const HashLayout &L, size_t &bytes) {
+    HashArgs a{};
+    char *p = static_cast<char *>(scratch);
+    size_t at = 0;
+    auto take = [&](size_t b) {
+        char *q = p ? p + at : nullptr;
+        at += round_up(b);
+        return q;
+    };
+    a.key = reinterpret_cast<unsigned long long *>(take((size_t)L.cap * 8));
+    a.first = reinterpret_cast<uint32_t *>(take(((size_t)L.cap + 2) * 4));
+    a.slot = reinterpret_cast<uint32_t *>(take(n * 4));
+    a.bcount = reinterpret_cast<uint32_t *>(take(((size_t)L.nb + 1) * 4 + scan_temp_bytes((size_t)L.nb + 1)));
+    a.bpos = reinterpret_cast<uint32_t *>(take(((size_t)L.cap + 1) * 4));
+    a.bkeys = reinterpret_cast<uint64_t *>(take(((size_t)L.cap + 1) * 8));
+    a.rank_of = reinterpret_cast<uint32_t *>(take(((size_t)L.cap + 2) * 4));
+    a.wg_range = reinterpret_cast<unsigned long long *>(take(((n + HROWS - 1) / HROWS) * 16));
+    a.st = reinterpret_cast<HashState *>(take(sizeof(HashState)));
+    a.n = (int64_t)n;
+    a.cap = L.cap;
+    a.nb = L.nb;
+    a.nb_log = L.nb_log;
+    bytes = at;
+    return a;
+}
+
+}  // namespace
+
+size_t rank_scratch_bytes(size_t n) {
+    size_t bytes = radix_bytes(n);
+    if ((int64_t)n >= HASH_FROM) {
+        size_t hb = 0;
+        hash_carve(nullptr, n, hash_layout(n), hb);
+        bytes = hb > bytes ? hb : bytes;
+    }
+    return bytes;
 }
 
 namespace {
@@ -707,6 +1078,45 @@ ef_status sort_u64(const uint64_t *words, const uint32_t *rows_in, size_t n, uin
     return EF_OK;
 }
 
+namespace {
+
+// Ranking by hashing (the kernels above); `done` false when this path
+// cannot finish for these words (the caller ranks them by sorting).
+ef_status rank_by_hash(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *first, size_t *n_ranks,
+                       void *scratch, bool &done, hipStream_t s) {
+    done = false;
+    const HashLayout L = hash_layout(n);
+    size_t bytes = 0;
+    HashArgs a = hash_carve(scratch, n, L, bytes);
+    a.words = words;
+    a.rank = rank;
+    a.first_out = first;
+    const unsigned slot_blocks = blocks_for((size_t)L.cap + 2, 256) < 4096 ? blocks_for((size_t)L.cap + 2, 256) : 4096;
+    hipLaunchKernelGGL(hash_clear_kernel, dim3(slot_blocks), dim3(256), 0, s, a);
+    const unsigned groups = (unsigned)((n + HROWS - 1) / HROWS);
+    hipLaunchKernelGGL(hash_insert_kernel, dim3(groups), dim3(HTHREADS), 0, s, a);
+    hipLaunchKernelGGL(hash_range_kernel, dim3(1), dim3(1024), 0, s, a, (int)groups);
+    hipLaunchKernelGGL(bucket_count_kernel, dim3(slot_blocks), dim3(256), 0, s, a);
+    EF_TRY(scan_u32(a.bcount, a.bcount, (size_t)L.nb + 1, a.bcount + L.nb + 1, s));
+    hipLaunchKernelGGL(bucket_scatter_kernel, dim3(slot_blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(bucket_rank_kernel, dim3(slot_blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(rank_rows_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+    EF_HIP(hipGetLastError());
+    struct {
+        HashState st;
+        uint32_t total;
+    } h;
+    EF_HIP(hipMemcpyAsync(&h.st, a.st, sizeof h.st, hipMemcpyDeviceToHost, s));
+    EF_HIP(hipMemcpyAsync(&h.total, a.bcount + L.nb, 4, hipMemcpyDeviceToHost, s));
+    EF_HIP(hipStreamSynchronize(s));
+    if (h.st.flags) return EF_OK;
+    if (n_ranks) *n_ranks = h.total;
+    done = true;
+    return EF_OK;
+}
+
+}  // namespace
+
 ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *first,
                    size_t *n_ranks, void *scratch, hipStream_t s) {
     if (n_ranks) *n_ranks = 0;
@@ -719,6 +1129,11 @@ ef_status rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *fi
     if (!scratch) {
         EF_TRY(own.alloc(rank_scratch_bytes(n)));
         scratch = own.ptr;
+    }
+    if ((int64_t)n >= HASH_FROM) {
+        bool done = false;
+        EF_TRY(rank_by_hash(words, n, rank, first, n_ranks, scratch, done, s));
+        if (done) return EF_OK;
     }
     const RadixScratch x = carve(scratch, n);
     const uint64_t *w_cur;

@@ -307,10 +307,12 @@ ef_status ef_dense_accumulate_f64(const double *a, const uint32_t *held, const d
 ef_status ef_dense_product_f32(const float *a, const float *b, float *out, size_t batches,
                                size_t m, size_t n, size_t k, uint32_t precision, void *stream);
 
-/* Radix ranking (discriminate.rs:103-156, 280-292): rank[i] is the number
- * of distinct words below words[i]; first[r] the lowest row holding the
- * r-th distinct word.  *n_ranks (host) receives the number of ranks.
- * `scratch` is device memory of ef_rank_scratch_bytes(n) bytes. */
+/* Ranking (discriminate.rs:103-156, 280-292): rank[i] is the number of
+ * distinct words below words[i]; first[r] the lowest row holding the r-th
+ * distinct word (first holds n entries).  *n_ranks (host) receives the
+ * number of ranks.  `scratch` is device memory of ef_rank_scratch_bytes(n)
+ * bytes.  From 65536 rows the words are deduplicated by hashing first;
+ * the radix sort ranks the rest and what hashing cannot finish. */
 size_t ef_rank_scratch_bytes(size_t n);
 ef_status ef_rank_u64(const uint64_t *words, size_t n, uint32_t *rank, uint32_t *first,
                       size_t *n_ranks, void *scratch, void *stream);

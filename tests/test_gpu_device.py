@@ -106,6 +106,41 @@ def test_ranks_constant_and_narrow_words(cuda):
     _check_ranks(w | np.uint64(0xFF00000000000001), cuda)
 
 
+def _hash_path_words(case: str) -> np.ndarray:
+    u = np.uint64
+    if case == "small_ints":            # words spread evenly over a narrow range
+        return data.next_u64(21, 1, 200_000) % u(1000)
+    if case == "all_distinct":          # more distinct words than the table lists: ranked by sorting
+        return data.hash64(np.arange(150_000, dtype=np.uint64), 9)
+    if case == "clustered":             # (hi << 32 | lo) pairs: buckets too large, ranked by sorting
+        hi = data.next_u64(22, 1, 200_000) % u(4)
+        lo = data.next_u64(22, 2, 200_000) % u(10_000)
+        return (hi << u(32)) | lo
+    if case == "extremes":              # 0, ~0 (the table's free-slot mark) and neighbours
+        vals = np.array([0, 1, 1 << 63, 0xFFFFFFFFFFFFFFFF, 0xFFFFFFFFFFFFFFFE], dtype=np.uint64)
+        return vals[data.next_u64(23, 1, 100_000) % u(5)]
+    if case == "all_empty_mark":
+        return np.full(70_000, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+    if case == "ragged_zipf":
+        return data.hash64(data.zipf_ids(24, 1, 65_536 + 777, 1 << 16, 1.1), 5)
+    if case == "below_threshold":
+        return data.hash64(data.next_u64(25, 1, 65_535) % u(5000), 1)
+    if case == "at_threshold":
+        return data.hash64(data.next_u64(25, 2, 65_536) % u(5000), 1)
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("case", ["small_ints", "all_distinct", "clustered", "extremes", "all_empty_mark",
+                                  "ragged_zipf", "below_threshold", "at_threshold"])
+def test_ranks_hashing_path(cuda, case):
+    """From 65536 rows ranking deduplicates through a hash table and ranks
+    the distinct words by bucket, falling back to the radix sort where that
+    cannot finish (too many distinct words, a crowded bucket): every case
+    must give the closed form's ranks and first rows."""
+    print(f"ranks hashing path {case}", flush=True)
+    _check_ranks(_hash_path_words(case), cuda)
+
+
 def test_ranks_zipf_keys(cuda):
     print("ranks zipf 2M", flush=True)
     ids = data.zipf_ids(5, 1, 2_000_000, 1 << 20, 1.1)
