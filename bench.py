@@ -48,11 +48,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the secondary lines (dense fold, ranking, batched and large folds, smoother, "
-                         "regridding)")
+                         "regridding, chain, sparse product, attention over Arrow, gradient memory)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one plain 256 MiB device copy first (a known-size kernel for counter scaling)")
     ap.add_argument("--line", choices=["dense_fold", "ranking", "batched_folds", "large_fold", "smoother",
-                                       "regridding", "chain", "sparse_product"],
+                                       "regridding", "chain", "sparse_product", "attention_arrow",
+                                       "gradient_memory"],
                     help="print one secondary line alone (for profiling one workload) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -77,7 +78,9 @@ def main():
                "batched_folds": lambda: batched_folds_line(dev), "large_fold": lambda: large_fold_line(dev),
                "smoother": smoother_line,
                "regridding": regrid_line, "chain": lambda: chain_line(not args.no_cpu),
-               "sparse_product": lambda: sparse_product_line(not args.no_cpu)}[args.line]
+               "sparse_product": lambda: sparse_product_line(not args.no_cpu),
+               "attention_arrow": attention_arrow_line,
+               "gradient_memory": lambda: gradient_memory_line(dev)}[args.line]
         print(json.dumps({args.line: one()}), flush=True)
         return
     q64, k64, v64 = data.attention_tables(H, T, D, seed=3)
@@ -177,7 +180,9 @@ def main():
                      "batched_folds": batched_folds_line(dev), "large_fold": large_fold_line(dev),
                      "smoother": smoother_line(),
                      "regridding": regrid_line(), "chain": chain_line(not args.no_cpu),
-                     "sparse_product": sparse_product_line(not args.no_cpu)}
+                     "sparse_product": sparse_product_line(not args.no_cpu),
+                     "attention_arrow": attention_arrow_line(),
+                     "gradient_memory": gradient_memory_line(dev)}
 
     line = {
         "metric": "fold GFLOP/s + attention fwd+bwd ms: mixed vs plain FP32, 1 MI355X",
@@ -521,6 +526,107 @@ def sparse_product_line(with_cpu: bool):
             raise SystemExit("sparse product: device groups or sums differ from the oracle's")
         line["bit_exact_vs_oracle"] = True
     return line
+
+
+def attention_pipeline_ir():
+    """Causal attention as the reference's fold pipeline over q(h, t, d, v),
+    k(h, t, d, v), v(h, t, d, v) (pipeline_vs_sql.rs:622-690): producer
+    dimensions 0 = tq, 1 = d, 2 = tk, 3 = h, the pipeline's own 4 = the
+    value's d; SUM(q.v * k.v) grouped by (h, tq, tk) where tq >= tk,
+    weights exp(s - MAX(s) OVER (h, tq)) over their sum, applied to v."""
+    from einfold_amd import ir
+    EQ = ir.KeyEquality.EQUAL
+    q = ir.Operand((ir.Key(3, 0), ir.Key(0, 1), ir.Key(1, 2)), 3)
+    k = ir.Operand((ir.Key(3, 0), ir.Key(2, 1), ir.Key(1, 2)), 3)
+    producer = ir.Fold([EQ] * 4, [q, k], [ir.Output(3), ir.Output(0), ir.Output(2)],
+                       conditions=[ir.Condition(0, ir.Comparison.GREATER_OR_EQUAL, 2, 0)])
+    values = ir.Operand((ir.Key(3, 0), ir.Key(2, 1), ir.Key(4, 2)), 3)
+    return ir.Pipeline(producer, ir.Exp(), [EQ], values, [3, 0, 4], ir.Normalizer.KEPT_GROUPS)
+
+
+def attention_arrow_line():
+    """Config 3 through the drop-in boundary: ef_pipeline on Arrow tables in
+    host memory (q, k and v 2M rows each: h, t, d INT32 and v FLOAT32, 8
+    heads x T = 4096 x d = 64), causal, the SQL's exp(s - max) weights,
+    Arrow out (2M rows); host-to-device and device-to-host transfers and
+    the Arrow import and export included.  BF16x3 and FP32, each against
+    the same call on the values widened to Float64 (the FP64 path)."""
+    import pyarrow as pa
+    from einfold_amd import data, kernels
+    q, k, v = data.attention_tables(H, T, D, seed=3)
+    q, k, v = (x.astype(np.float32) for x in (q, k, v))
+    t32 = data.attention_arrow(q, k, v, pa.float32())
+    t64 = data.attention_arrow(q.astype(np.float64), k.astype(np.float64), v.astype(np.float64), pa.float64())
+    p = attention_pipeline_ir()
+
+    def values(piped):
+        return np.concatenate([b.column(b.num_columns - 1).to_numpy() for b in piped.batches])
+
+    want = values(kernels.pipeline(p, *t64))
+    line = {"workload": "causal attention pipeline over Arrow, 8 heads x T=4096 x d=64 (2M rows per operand)",
+            "rows_out": int(want.size)}
+    for prec in ("bf16x3", "fp32"):
+        opts = kernels.Options(accept_float32=True, precision=prec)
+        got = kernels.pipeline(p, *t32, opts)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            got = kernels.pipeline(p, *t32, opts)
+            times.append((time.perf_counter() - t0) * 1e3)
+        g = values(got)
+        line[f"ms_arrow_to_arrow_{prec}"] = round(sorted(times)[1], 2)
+        line[f"rel_error_vs_fp64_{prec}"] = float(np.linalg.norm(g - want) / np.linalg.norm(want))
+        line["tiles_visited"], line["tiles"] = got.tiles_visited, got.tiles
+    return line
+
+
+def gradient_memory_line(dev):
+    """Config 4's memory report: device memory of the gradient step (forward,
+    then dq, dk, dv of SUM(out * g)) at 4 heads x T = 2048 x d = 64, causal,
+    beyond the inputs: the recomputing backward (saved row max and row sum,
+    its scratch included) against a stored-scores variant (torch's FP32
+    autograd of the same causal attention, which keeps the T x T weights of
+    every head).  Peaks from torch's allocator, which also serves the
+    library's scratch."""
+    import torch
+    from einfold_amd import data, device
+    hh, tt, dd = 4, 2048, 64
+    q64, k64, v64 = data.attention_tables(hh, tt, dd, seed=4)
+    g64 = data.normal(4, 4, hh * tt * dd).reshape(hh, tt, dd)
+    q, k, v, g = (torch.from_numpy(np.ascontiguousarray(x)).float().to(dev) for x in (q64, k64, v64, g64))
+    torch.cuda.synchronize()
+
+    def peak(fn):
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        keep = fn()
+        torch.cuda.synchronize()
+        used = torch.cuda.max_memory_allocated(dev) - base
+        del keep
+        torch.cuda.empty_cache()
+        return used
+
+    def recompute():
+        out, m, l = device.attention_fwd(q, k, v, SCALE, causal=True, precision="bf16x3")
+        return out, device.attention_bwd(q, k, v, out, g, m, l, SCALE, causal=True, precision="bf16x3")
+
+    def stored():
+        qq, kk, vv = (x.clone().requires_grad_() for x in (q, k, v))
+        s = torch.matmul(qq, kk.transpose(1, 2).contiguous()) * SCALE
+        mask = torch.ones(tt, tt, dtype=torch.bool, device=dev).tril()
+        p = torch.softmax(s.masked_fill(~mask, float("-inf")), dim=-1)
+        out = torch.matmul(p, vv)
+        (out * g).sum().backward()
+        return out.detach(), qq.grad, kk.grad, vv.grad
+
+    mib = lambda b: round(b / 2 ** 20, 2)
+    r, s = peak(recompute), peak(stored)
+    return {"workload": "attention gradient step, 4 heads x T=2048 x d=64, causal (FP32 inputs)",
+            "recompute_peak_mib": mib(r), "stored_scores_peak_mib": mib(s),
+            "saved_state_mib": {"recompute_row_stats": mib(2 * 4 * hh * tt),
+                                "stored_scores": mib(4 * hh * tt * tt)},
+            "ratio": round(s / max(r, 1), 1)}
 
 
 def ranking_line(dev, with_cpu: bool = False):

@@ -1776,15 +1776,44 @@ bool chain_guarded(const FoldModel &m, const std::vector<const Relation *> &ops)
             const ColumnView &v = r.batches[b][c];
             rows += r.batch_rows[b];
             const double *x = reinterpret_cast<const double *>(v.values);
-            for (int64_t i = 0; i < v.length; i++) {
-                if (!v.valid(i)) continue;
-                if (!std::isfinite(x[i])) return false;
-                largest = std::max(largest, std::fabs(x[i]));
+            // Branch-free passes (the compiler vectorizes them): |x| <= the
+            // largest finite double fails for inf and NaN alike.
+            bool finite = true;
+            double big = 0.0;
+            if (v.null_count == 0) {
+                for (int64_t i = 0; i < v.length; i++) {
+                    const double a = std::fabs(x[i]);
+                    finite &= a <= std::numeric_limits<double>::max();
+                    big = a > big ? a : big;
+                }
+            } else {
+                for (int64_t i = 0; i < v.length; i++) {
+                    const double a = v.valid(i) ? std::fabs(x[i]) : 0.0;
+                    finite &= a <= std::numeric_limits<double>::max();
+                    big = a > big ? a : big;
+                }
             }
+            if (!finite) return false;
+            largest = std::max(largest, big);
         }
         bound *= largest * (double)rows;
     }
     return bound <= CHAIN_LARGEST_BOUND;
+}
+
+// The largest key of a column of integer type T, and whether one is
+// negative (unsigned 64-bit keys of 2^63 and above count as negative: too
+// large for i64), branch-free.
+template <class T> void key_range(const T *k, int64_t n, int64_t &hi, bool &negative) {
+    int64_t h = -1;
+    bool neg = false;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t x = (int64_t)k[i];
+        neg |= x < 0;
+        h = x > h ? x : h;
+    }
+    hi = h;
+    negative = neg;
 }
 
 // dense.rs:97-111 for one batch's key column: integers without NULLs, each
@@ -1792,26 +1821,18 @@ bool chain_guarded(const FoldModel &m, const std::vector<const Relation *> &ops)
 bool chain_key_extent(const ColumnView &v, int64_t &extent) {
     if (!is_integer(v.type) || v.null_count > 0) return false;
     int64_t hi = -1;
-    for (int64_t i = 0; i < v.length; i++) {
-        int64_t k;
-        switch (v.type) {
-        case Type::Int8: k = reinterpret_cast<const int8_t *>(v.values)[i]; break;
-        case Type::UInt8: k = reinterpret_cast<const uint8_t *>(v.values)[i]; break;
-        case Type::Int16: k = reinterpret_cast<const int16_t *>(v.values)[i]; break;
-        case Type::UInt16: k = reinterpret_cast<const uint16_t *>(v.values)[i]; break;
-        case Type::Int32: k = reinterpret_cast<const int32_t *>(v.values)[i]; break;
-        case Type::UInt32: k = reinterpret_cast<const uint32_t *>(v.values)[i]; break;
-        case Type::Int64: k = reinterpret_cast<const int64_t *>(v.values)[i]; break;
-        default: {
-            const uint64_t u = reinterpret_cast<const uint64_t *>(v.values)[i];
-            if (u >> 63) return false;   // too large for i64
-            k = (int64_t)u;
-        }
-        }
-        if (k < 0) return false;
-        hi = std::max(hi, k);
+    bool negative = false;
+    switch (v.type) {
+    case Type::Int8: key_range(reinterpret_cast<const int8_t *>(v.values), v.length, hi, negative); break;
+    case Type::UInt8: key_range(reinterpret_cast<const uint8_t *>(v.values), v.length, hi, negative); break;
+    case Type::Int16: key_range(reinterpret_cast<const int16_t *>(v.values), v.length, hi, negative); break;
+    case Type::UInt16: key_range(reinterpret_cast<const uint16_t *>(v.values), v.length, hi, negative); break;
+    case Type::Int32: key_range(reinterpret_cast<const int32_t *>(v.values), v.length, hi, negative); break;
+    case Type::UInt32: key_range(reinterpret_cast<const uint32_t *>(v.values), v.length, hi, negative); break;
+    case Type::Int64: key_range(reinterpret_cast<const int64_t *>(v.values), v.length, hi, negative); break;
+    default: key_range(reinterpret_cast<const uint64_t *>(v.values), v.length, hi, negative); break;
     }
-    if (hi == std::numeric_limits<int64_t>::max()) return false;
+    if (negative || hi == std::numeric_limits<int64_t>::max()) return false;
     extent = hi + 1;
     return true;
 }
