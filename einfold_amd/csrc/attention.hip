@@ -106,6 +106,7 @@ __device__ __forceinline__ void put_split(typename Elem<P>::T *base, int64_t pla
 // rows of one batch.
 // ---------------------------------------------------------------------
 struct Counter { unsigned next, done; };
+constexpr int QUEUES = 8;   // work counters per kernel (one per XCD residue)
 
 // One prep launch: blockIdx.z picks a job.  Job z < njobs prepares one
 // operand; job z == njobs (when out is set) computes delta[b][x] = sum_d
@@ -180,13 +181,23 @@ __global__ __launch_bounds__(256) void prep_kernel(PrepArgs a) {
 // launch (plain stores, so the pair may be captured in a graph).
 // ---------------------------------------------------------------------
 __global__ void zero_counters_kernel(Counter *c) {
-    if (threadIdx.x < 4) { c[threadIdx.x].next = 0; c[threadIdx.x].done = 0; }
+    if (threadIdx.x < QUEUES) { c[threadIdx.x].next = 0; c[threadIdx.x].done = 0; }
 }
 
 // A workgroup's items: its first is blockIdx.x, the rest (when the grid
-// is smaller than the item count) come from the counter, offset by the
-// grid.  A grid of one workgroup per item never touches the counter: the
+// is smaller than the item count) come from a counter, offset by the grid.
+// A grid of one workgroup per item never touches the counters: the
 // hardware hands out workgroups in index order, heaviest items first.
+//
+// Items are numbered batch fastest (item % batches is the batch), and the
+// hardware deals workgroups to the eight XCDs in turn (workgroup w to XCD
+// w % 8), so with a multiple of eight batches each XCD's L2 holds the
+// streamed tiles of its own batches only.  Later items keep that: a grid
+// that is a multiple of eight takes them from one counter per residue of
+// the item number mod 8 (QUEUES counters), item grid + r + 8 n for the
+// n-th taken by the workgroups w = r mod 8.  Every item is taken once
+// whichever XCD a workgroup really runs on: the residues only pick
+// which items share an L2.
 struct ItemQueue {
     Counter *ctr;
     int *slot;
@@ -201,8 +212,17 @@ struct ItemQueue {
         __syncthreads();
         const int item = *slot;
         __syncthreads();   // all have read slot before thread 0 writes it again
-        if (threadIdx.x == 0)
-            *slot = (int)gridDim.x >= items ? items : (int)(gridDim.x + atomicAdd(&ctr->next, 1u));
+        if (threadIdx.x == 0) {
+            const int grid = (int)gridDim.x;
+            if (grid >= items) {
+                *slot = items;
+            } else {
+                const int stride = grid % QUEUES == 0 ? QUEUES : 1;
+                const int r = (int)blockIdx.x % stride;
+                const long long nxt = (long long)grid + r + (long long)stride * atomicAdd(&ctr[r].next, 1u);
+                *slot = nxt < items ? (int)nxt : items;
+            }
+        }
         return item;
     }
 };
@@ -367,10 +387,10 @@ void attn_fwd_kernel(FwdArgs a) {
     for (;;) {
         const int item = queue.next();
         if (item >= a.items) break;
-        const int part = item % a.parts;
-        const int64_t blk = item / a.parts;
-        const int64_t b = blk % a.batches;
-        const int64_t qt = a.reverse ? nqt - 1 - blk / a.batches : blk / a.batches;
+        const int64_t b = item % a.batches;
+        const int64_t rest = item / a.batches;
+        const int part = (int)(rest % a.parts);
+        const int64_t qt = a.reverse ? nqt - 1 - rest / a.parts : rest / a.parts;
         const int64_t q0 = qt * BW;
         const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
         const E *K = static_cast<const E *>(a.kr) + b * a.tkp * D;
@@ -770,10 +790,10 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     for (;;) {
         const int item = queue.next();
         if (item >= a.items) break;
-        const int part = item % a.parts;
-        const int64_t blk = item / a.parts;
-        const int64_t b = blk % a.batches;
-        const int64_t kt = a.reverse ? nkt - 1 - blk / a.batches : blk / a.batches;
+        const int64_t b = item % a.batches;
+        const int64_t rest = item / a.batches;
+        const int part = (int)(rest % a.parts);
+        const int64_t kt = a.reverse ? nkt - 1 - rest / a.parts : rest / a.parts;
         const int64_t k0 = kt * BW;
         const int64_t kvalid = (k0 + BW < a.tk ? k0 + BW : a.tk);   // keys past tk are padding
         const In *K = static_cast<const In *>(a.k) + b * a.tk * D;
@@ -1034,10 +1054,10 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
     for (;;) {
         const int item = queue.next();
         if (item >= a.items) break;
-        const int part = item % a.parts;
-        const int64_t blk = item / a.parts;
-        const int64_t b = blk % a.batches;
-        const int64_t qt = a.reverse ? nqt - 1 - blk / a.batches : blk / a.batches;
+        const int64_t b = item % a.batches;
+        const int64_t rest = item / a.batches;
+        const int part = (int)(rest % a.parts);
+        const int64_t qt = a.reverse ? nqt - 1 - rest / a.parts : rest / a.parts;
         const int64_t q0 = qt * BW;
         const In *Q = static_cast<const In *>(a.q) + b * a.tq * D;
         const In *DO = static_cast<const In *>(a.dout) + b * a.tq * D;
@@ -1326,7 +1346,7 @@ ef_status run_fwd(const ef_attention_desc &d, const void *q, const void *k, cons
     pa.njobs = 2;
     pa.B = B;
     pa.zero = reinterpret_cast<Counter *>(scratch + L.counters);
-    pa.nzero = 1;
+    pa.nzero = QUEUES;
     hipLaunchKernelGGL((prep_kernel<P, D>), dim3((unsigned)(tkp / 64), (unsigned)B, 2), dim3(256), 0, s, pa);
     FwdArgs a;
     a.q = q; a.kr = kr; a.vt = vt; a.out = out;
@@ -1404,7 +1424,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         pa.njobs = 4;
         pa.B = B;
         pa.zero = reinterpret_cast<Counter *>(scratch + L.counters);
-        pa.nzero = 2;
+        pa.nzero = 2 * QUEUES;
         pa.out = out;
         pa.dout = dout;
         pa.delta = scratch + L.delta;
@@ -1429,7 +1449,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         }
     }
     if (stages & 4) {
-        a.ctr = reinterpret_cast<Counter *>(scratch + L.counters) + 1;
+        a.ctr = reinterpret_cast<Counter *>(scratch + L.counters) + QUEUES;
         const int64_t blocks = B * ((tq + BW - 1) / BW);
         a.parts = parts_for(d, blocks);
         a.items = (int)(blocks * a.parts);
