@@ -1,6 +1,9 @@
 // Arrow C Data Interface import and export (see arrow_io.h).
 #include "arrow_io.h"
 
+#include <map>
+#include <mutex>
+
 #include <cstdlib>
 #include <cstring>
 
@@ -113,22 +116,63 @@ bool read_relation(const ef_relation *in, Relation &out) {
     return true;
 }
 
-// Output buffers from 1 MiB up are pinned (hipHostMalloc): the results'
-// device-to-host copies then run as direct DMA into them, where a fresh
-// pageable buffer would be staged in chunks and take a page fault on first
-// touch of each page (about 13x slower for hundreds of MB).  Small ones
-// stay in pageable memory, where pinning would cost more than it saves.
-constexpr size_t PIN_FROM = (size_t)1 << 20;
+// Output buffers from 64 KiB up are pinned: the results' device-to-host
+// copies then run as direct DMA into them, where a pageable buffer would be
+// staged in chunks (and, fresh, take a page fault on first touch of each
+// page: about 13x slower for hundreds of MB).  Pinning a buffer
+// (hipHostMalloc) costs far more than copying into it, so released pinned
+// buffers are kept in a pool by size class and handed out again, up
+// to PIN_POOL_CAP bytes; the pool lives until the process ends (its
+// buffers are never freed after the runtime has shut down).  Smaller
+// buffers stay pageable.
+namespace {
+constexpr size_t PIN_FROM = (size_t)64 << 10;
+constexpr size_t PIN_POOL_CAP = (size_t)1 << 30;
+
+struct PinnedPool {
+    std::mutex mu;
+    std::multimap<size_t, void *> free;   // size class -> buffer
+    size_t cached = 0;
+};
+
+PinnedPool &pinned_pool() {
+    static PinnedPool *pool = new PinnedPool;
+    return *pool;
+}
+
+// Size classes: powers of two up to 1 MiB, then multiples of an eighth of
+// the power of two below (at most 12.5 % over the size asked).
+size_t size_class(size_t n) {
+    size_t c = PIN_FROM;
+    while (c < n) c <<= 1;
+    if (c <= ((size_t)1 << 20)) return c;
+    const size_t step = (c >> 1) / 8;
+    return (n + step - 1) / step * step;
+}
+}  // namespace
 
 HostBuffer::HostBuffer(size_t n) : bytes(n) {
     const size_t padded = ((n + 63) / 64) * 64 + 64;
     if (padded >= PIN_FROM) {
-        void *p = nullptr;
-        if (hipHostMalloc(&p, padded, hipHostMallocDefault) == hipSuccess && p) {
-            data = static_cast<uint8_t *>(p);
+        const size_t cls = size_class(padded);
+        PinnedPool &pool = pinned_pool();
+        {
+            std::lock_guard<std::mutex> lock(pool.mu);
+            auto it = pool.free.find(cls);
+            if (it != pool.free.end()) {
+                data = static_cast<uint8_t *>(it->second);
+                pool.cached -= cls;
+                pool.free.erase(it);
+            }
+        }
+        if (!data) {
+            void *p = nullptr;
+            if (hipHostMalloc(&p, cls, hipHostMallocDefault) == hipSuccess && p) data = static_cast<uint8_t *>(p);
+            else (void)hipGetLastError();
+        }
+        if (data) {
             pinned = true;
-        } else {
-            (void)hipGetLastError();
+            capacity = cls;
         }
     }
     if (!data) data = static_cast<uint8_t *>(std::aligned_alloc(64, padded));
@@ -137,8 +181,20 @@ HostBuffer::HostBuffer(size_t n) : bytes(n) {
 }
 
 HostBuffer::~HostBuffer() {
-    if (pinned) (void)hipHostFree(data);
-    else std::free(data);
+    if (!pinned) {
+        std::free(data);
+        return;
+    }
+    PinnedPool &pool = pinned_pool();
+    {
+        std::lock_guard<std::mutex> lock(pool.mu);
+        if (pool.cached + capacity <= PIN_POOL_CAP) {
+            pool.free.emplace(capacity, data);
+            pool.cached += capacity;
+            return;
+        }
+    }
+    (void)hipHostFree(data);
 }
 
 namespace {

@@ -52,10 +52,12 @@ struct Relation {
 // Reads an ef_relation; false (with the error set) if malformed.
 bool read_relation(const ef_relation *in, Relation &out);
 
-// Host buffer aligned to 64 bytes, as Arrow recommends; pinned when large.
+// Host buffer aligned to 64 bytes, as Arrow recommends; pinned (from a
+// pool of released pinned buffers) from 64 KiB.
 struct HostBuffer {
     uint8_t *data = nullptr;
     size_t bytes = 0;
+    size_t capacity = 0;   // a pinned buffer's size class
     bool pinned = false;
     explicit HostBuffer(size_t n);
     ~HostBuffer();

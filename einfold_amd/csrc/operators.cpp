@@ -518,6 +518,7 @@ ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation 
     uint32_t *held, *present;
     uint8_t *reached, *valued = nullptr, *va, *vb;
     PlaceFlags *flags;
+    c.begin_zeros();
     EF_TRY(c.zeros(am, na));
     EF_TRY(c.zeros(held, na));
     EF_TRY(c.zeros(reached, nx));
@@ -525,6 +526,7 @@ ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation 
     EF_TRY(c.zeros(bm, nb));
     EF_TRY(c.zeros(present, nb));
     EF_TRY(c.zeros(flags, 1));
+    EF_TRY(c.end_zeros());
     EF_TRY(c.alloc(fa, (size_t)a.rows));
     EF_TRY(c.alloc(va, (size_t)a.rows));
     EF_TRY(c.alloc(fb, (size_t)b.rows));

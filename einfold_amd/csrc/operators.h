@@ -74,6 +74,9 @@ struct Call {
     hipStream_t stream = nullptr;
     std::deque<DeviceBuffer> buffers;   // the chunks
     size_t used = 0;                    // bytes carved from the last chunk
+    bool zero_run = false, zero_split = false;
+    size_t zero_chunk = 0;
+    std::vector<std::pair<void *, size_t>> pending_zeros;
 
     explicit Call(const ef_options *o) : options(o), stream(as_stream(o ? o->stream : nullptr)) {}
     ~Call() {
@@ -121,9 +124,37 @@ struct Call {
         p = static_cast<T *>(v);
         return st;
     }
+    // n zeroed T.  Between begin_zeros() and end_zeros() the buffers are
+    // only carved, and end_zeros() clears them with one fill when they lie
+    // in one chunk (carved back to back), else one fill each.
     template <class T> ef_status zeros(T *&p, size_t n) {
         EF_TRY(alloc(p, n));
+        if (zero_run) {
+            pending_zeros.push_back({p, n * sizeof(T) + 16});
+            if (buffers.size() != zero_chunk) zero_split = true;
+            return EF_OK;
+        }
         EF_HIP(hipMemsetAsync(p, 0, n * sizeof(T) + 16, stream));
+        return EF_OK;
+    }
+    void begin_zeros() {
+        zero_run = true;
+        zero_split = false;
+        zero_chunk = buffers.size();
+        pending_zeros.clear();
+    }
+    ef_status end_zeros() {
+        zero_run = false;
+        if (pending_zeros.empty()) return EF_OK;
+        if (!zero_split && zero_chunk == buffers.size()) {
+            char *lo = static_cast<char *>(pending_zeros.front().first);
+            const auto &last = pending_zeros.back();
+            const size_t span = static_cast<char *>(last.first) + last.second - lo;
+            EF_HIP(hipMemsetAsync(lo, 0, span, stream));
+        } else {
+            for (auto &z : pending_zeros) EF_HIP(hipMemsetAsync(z.first, 0, z.second, stream));
+        }
+        pending_zeros.clear();
         return EF_OK;
     }
     template <class T> ef_status upload(T *dst, const void *src, size_t n) {
