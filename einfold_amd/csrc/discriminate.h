@@ -64,6 +64,13 @@ ef_status gather_sorted(const uint32_t *rows, const uint64_t *w, int64_t n, cons
                         const double *value, uint32_t *low, uint8_t *valid_s, double *value_s, hipStream_t s);
 ef_status group_bounds(const uint32_t *a_start, const uint32_t *a_z, const uint32_t *b_start, int64_t nx, int64_t ny,
                        uint64_t *bound, unsigned long long *pairs, hipStream_t s);
+// Gustavson's product for operands with heavy join ranks: g.acc and
+// g.state hold one zeroed cell per (x, y) (nx * ny), heavy_host/heavy_dev
+// the heavy z ranks in ascending order (host and device copies),
+// heavy_rows their b bucket sizes, split 2 * n_heavy * nx words of
+// scratch.  Leaves each x's groups staged as gustavson does.
+ef_status gustavson_heavy(const GustavsonArgs &g, const uint32_t *heavy_host, const uint32_t *heavy_dev,
+                          const uint32_t *heavy_rows, int n_heavy, uint32_t *split, hipStream_t s);
 ef_status gustavson(const GustavsonArgs &g, int blocks, hipStream_t s);
 ef_status widen_counts(const uint32_t *in, int64_t n, uint64_t *out, hipStream_t s);
 ef_status emit_groups(int64_t nx, const uint64_t *bound_off, const uint64_t *out_off, const uint32_t *g_count,

@@ -346,6 +346,141 @@ __global__ __launch_bounds__(256) void gustavson_lds_kernel(GustavsonArgs g) {
         }
     }
 }
+// ---- Gustavson's product with heavy join ranks -------------------------
+//
+// Zipf-distributed join keys put most joined pairs in a few z buckets (the
+// hottest j of 100k rows per side holds two thirds of them).  Walking such
+// a bucket once per row of a, as above, is a read-modify-write of the
+// accumulator per pair.  Here every (x, y) cell has its own accumulator and
+// state in HBM, and the product runs in phases in z order: the light z's
+// below the first heavy one (a wave per x, as above), the first heavy z (a
+// thread per (x, run of equal y in its bucket): the cell's sum in a
+// register over the x's rows of a at that z, each times the run's rows of
+// b, both in order), the light z's up to the next heavy one, and so on.
+// Every cell therefore adds the same products in the same order as the
+// wave-per-x loop: z ascending, then a's rows, then b's rows.  The groups
+// come out by a scan of each x's states in y order.
+
+// split[2 k][x], split[2 k + 1][x]: the first row of x's a rows whose z is
+// at least heavy[k], and above it (a is sorted by (x, z)).
+__global__ __launch_bounds__(256) void heavy_split_kernel(const uint32_t *a_start, const uint32_t *a_z, int64_t nx,
+                                                          const uint32_t *heavy, int n_heavy, uint32_t *split) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    const uint32_t lo0 = a_start[x], hi0 = a_start[x + 1];
+    for (int k = 0; k < n_heavy; k++) {
+        const uint32_t h = heavy[k];
+        uint32_t lo = lo0, hi = hi0;
+        while (lo < hi) {   // first z >= h
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a_z[mid] < h) lo = mid + 1;
+            else hi = mid;
+        }
+        uint32_t e = lo, eh = hi0;
+        while (e < eh) {   // first z > h
+            const uint32_t mid = (e + eh) >> 1;
+            if (a_z[mid] <= h) e = mid + 1;
+            else eh = mid;
+        }
+        split[(2 * (int64_t)k) * nx + x] = lo;
+        split[(2 * (int64_t)k + 1) * nx + x] = e;
+    }
+}
+
+// The light z's of a phase: a wave per x over its a rows [lo[x], hi[x])
+// (null: the x's first or last row), into the x's own cells.
+__global__ __launch_bounds__(256) void gustavson_light_kernel(GustavsonArgs g, const uint32_t *lo_idx,
+                                                              const uint32_t *hi_idx) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t x = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); x < g.nx; x += waves) {
+        double *acc = g.acc + x * g.ny;
+        uint8_t *state = g.state + x * g.ny;
+        const uint32_t i0 = lo_idx ? lo_idx[x] : g.a_start[x];
+        const uint32_t i1 = hi_idx ? hi_idx[x] : g.a_start[x + 1];
+        for (uint32_t i = i0; i < i1; i++) {
+            const uint32_t z = g.a_z[i];
+            const bool av = g.a_valid[i] != 0;
+            const double p = g.a_value[i];
+            const uint32_t bs = g.b_start[z], be = g.b_start[z + 1];
+            for (uint32_t j = bs + lane; j < be; j += 64) {
+                const uint32_t y = g.b_y[j];
+                if (j != bs && g.b_y[j - 1] == y) continue;   // not the head of its run
+                uint8_t st = state[y] | 1;
+                double a = acc[y];
+                for (uint32_t k = j; k < be && g.b_y[k] == y; k++) {
+                    if (av && g.b_valid[k]) {
+                        a = a + p * g.b_value[k];
+                        st |= 2;
+                    }
+                }
+                acc[y] = a;
+                state[y] = st;
+            }
+            // This row's sums land before the next row's lanes read them.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// One heavy z: blocks of 256 rows of its bucket, per x (blockIdx.x =
+// x * per_x + block): the thread on a run's first row sums the run against
+// the x's a rows at z.
+__global__ __launch_bounds__(256) void gustavson_heavy_kernel(GustavsonArgs g, uint32_t h, const uint32_t *lo_idx,
+                                                              const uint32_t *hi_idx, int64_t per_x) {
+    const int64_t x = (int64_t)blockIdx.x / per_x;
+    const uint32_t i0 = lo_idx[x], i1 = hi_idx[x];
+    if (i0 == i1) return;
+    const uint32_t bs = g.b_start[h], be = g.b_start[h + 1];
+    const int64_t j = (int64_t)bs + ((int64_t)blockIdx.x % per_x) * 256 + threadIdx.x;
+    if (j >= be) return;
+    const uint32_t y = g.b_y[j];
+    if (j != bs && g.b_y[j - 1] == y) return;   // not the head of its run
+    double *cell = g.acc + x * g.ny + y;
+    uint8_t *cst = g.state + x * g.ny + y;
+    double a = *cell;
+    uint8_t st = *cst | 1;
+    for (uint32_t i = i0; i < i1; i++) {
+        const bool av = g.a_valid[i] != 0;
+        const double p = g.a_value[i];
+        for (int64_t k = j; k < be && g.b_y[k] == y; k++) {
+            if (av && g.b_valid[k]) {
+                a = a + p * g.b_value[k];
+                st |= 2;
+            }
+        }
+    }
+    *cell = a;
+    *cst = st;
+}
+
+// Each x's groups from its cells' states, in y order, into its staged
+// span (bound_off[x]), as the wave-per-x loop leaves them.
+__global__ __launch_bounds__(256) void collect_groups_kernel(GustavsonArgs g) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t x = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); x < g.nx; x += waves) {
+        const double *acc = g.acc + x * g.ny;
+        const uint8_t *state = g.state + x * g.ny;
+        const uint64_t base = g.bound_off[x];
+        uint32_t placed = 0;
+        for (int64_t y0 = 0; y0 < g.ny; y0 += 64) {
+            const int64_t y = y0 + lane;
+            const uint8_t st = y < g.ny ? state[y] : 0;
+            const uint64_t m = __ballot(st & 1);
+            if (st & 1) {
+                const uint64_t o = base + placed + __popcll(m & below);
+                g.g_y[o] = (uint32_t)y;
+                g.g_sum[o] = acc[y];
+                g.g_valued[o] = (st >> 1) & 1;
+            }
+            placed += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) g.g_count[x] = placed;
+    }
+}
 #pragma clang fp contract(on)
 
 // ---- output ----------------------------------------------------------------
@@ -608,6 +743,30 @@ ef_status gustavson(const GustavsonArgs &g, int blocks, hipStream_t s) {
     } else {
         hipLaunchKernelGGL(gustavson_kernel, dim3(blocks), dim3(256), 0, s, g);
     }
+    EF_HIP(hipGetLastError());
+    return EF_OK;
+}
+
+ef_status gustavson_heavy(const GustavsonArgs &g, const uint32_t *heavy_host, const uint32_t *heavy_dev,
+                          const uint32_t *heavy_rows, int n_heavy, uint32_t *split, hipStream_t s) {
+    if (g.nx == 0) return EF_OK;
+    const unsigned wave_blocks = (unsigned)((g.nx + 3) / 4 < 4096 ? (g.nx + 3) / 4 : 4096);
+    EF_LAUNCH(heavy_split_kernel, g.nx, g.a_start, g.a_z, g.nx, heavy_dev, n_heavy, split);
+    for (int k = 0; k <= n_heavy; k++) {
+        const uint32_t *lo = k == 0 ? nullptr : split + (2 * (int64_t)(k - 1) + 1) * g.nx;
+        const uint32_t *hi = k == n_heavy ? nullptr : split + (2 * (int64_t)k) * g.nx;
+        hipLaunchKernelGGL(gustavson_light_kernel, dim3(wave_blocks), dim3(256), 0, s, g, lo, hi);
+        if (k == n_heavy) break;
+        const int64_t per_x = ((int64_t)heavy_rows[k] + 255) / 256;
+        if (per_x == 0) continue;
+        if (per_x * g.nx > 0x7fffffffLL) {
+            set_error("a heavy join rank's bucket is too large for one launch");
+            return EF_INVALID;
+        }
+        hipLaunchKernelGGL(gustavson_heavy_kernel, dim3((unsigned)(per_x * g.nx)), dim3(256), 0, s, g, heavy_host[k],
+                           split + (2 * (int64_t)k) * g.nx, split + (2 * (int64_t)k + 1) * g.nx, per_x);
+    }
+    hipLaunchKernelGGL(collect_groups_kernel, dim3(wave_blocks), dim3(256), 0, s, g);
     EF_HIP(hipGetLastError());
     return EF_OK;
 }

@@ -1059,9 +1059,36 @@ ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Re
     EF_TRY(c.alloc(g_valued, staged));
     EF_TRY(c.alloc(g_count, (size_t)nx));
 
-    // Gustavson's product: wave slots, each with an accumulator of ny.
     size_t free_b = 0, total_b = 0;
     EF_HIP(hipMemGetInfo(&free_b, &total_b));
+    // Heavy join ranks: where every (x, y) cell fits in HBM and the groups
+    // fill at least a quarter of them, the z's whose b bucket holds
+    // HEAVY_ROWS rows or more (the HEAVY_MAX largest) run as dense phases
+    // (gustavson_heavy): the same sums in the same order.
+    constexpr uint32_t HEAVY_ROWS = 1024;
+    constexpr size_t HEAVY_MAX = 16;
+    std::vector<uint32_t> heavy, heavy_rows;
+    const double cells = (double)nx * (double)ny;
+    if (ny > 512 && nz > 0 && cells * 9 <= std::min((double)free_b / 2, 8.0 * (double)(1ull << 30)) &&
+        groups >= cells / 4) {
+        std::vector<uint32_t> bst((size_t)nz + 1);
+        EF_TRY(c.download<uint32_t>(bst.data(), b_start, (size_t)nz + 1));
+        EF_TRY(c.sync());
+        std::vector<std::pair<uint32_t, uint32_t>> big;   // (rows, z)
+        for (uint64_t z = 0; z < (uint64_t)nz; z++)
+            if (bst[z + 1] - bst[z] >= HEAVY_ROWS) big.push_back({bst[z + 1] - bst[z], (uint32_t)z});
+        std::sort(big.begin(), big.end(), [](const auto &l, const auto &r) {
+            return l.first != r.first ? l.first > r.first : l.second < r.second;
+        });
+        if (big.size() > HEAVY_MAX) big.resize(HEAVY_MAX);
+        std::sort(big.begin(), big.end(), [](const auto &l, const auto &r) { return l.second < r.second; });
+        for (auto &h : big) {
+            heavy.push_back(h.second);
+            heavy_rows.push_back(h.first);
+        }
+    }
+
+    // Gustavson's product: wave slots, each with an accumulator of ny.
     const double per_slot = (double)ny * 9.0;
     int64_t slots = std::min<int64_t>(2048, (int64_t)std::max(4.0, std::floor((double)free_b / 4 / per_slot)));
     slots = std::min<int64_t>(slots, ((nx + 3) / 4) * 4);
@@ -1077,15 +1104,27 @@ ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Re
     g.b_value = b_value;
     g.nx = nx;
     g.ny = ny;
-    EF_TRY(c.zeros(g.acc, (size_t)(slots * ny)));
-    EF_TRY(c.zeros(g.state, (size_t)(slots * ny)));
-    EF_TRY(c.zeros(g.next, 1));
     g.bound_off = bound_off;
     g.g_y = g_y;
     g.g_sum = g_sum;
     g.g_valued = g_valued;
     g.g_count = g_count;
-    EF_TRY(gustavson(g, (int)(slots / 4), c.stream));
+    if (!heavy.empty()) {
+        c.begin_zeros();
+        EF_TRY(c.zeros(g.acc, (size_t)(nx * ny)));
+        EF_TRY(c.zeros(g.state, (size_t)(nx * ny)));
+        EF_TRY(c.end_zeros());
+        uint32_t *heavy_dev, *split;
+        EF_TRY(c.alloc(heavy_dev, heavy.size()));
+        EF_TRY(c.upload<uint32_t>(heavy_dev, heavy.data(), heavy.size()));
+        EF_TRY(c.alloc(split, 2 * heavy.size() * (size_t)nx));
+        EF_TRY(gustavson_heavy(g, heavy.data(), heavy_dev, heavy_rows.data(), (int)heavy.size(), split, c.stream));
+    } else {
+        EF_TRY(c.zeros(g.acc, (size_t)(slots * ny)));
+        EF_TRY(c.zeros(g.state, (size_t)(slots * ny)));
+        EF_TRY(c.zeros(g.next, 1));
+        EF_TRY(gustavson(g, (int)(slots / 4), c.stream));
+    }
 
     if (!m.has_domain) {
         uint64_t *counts64, *out_off;

@@ -159,6 +159,37 @@ def test_config2_sparse_product_at_1e5_rows_deterministic_across_batches(cuda):
     assert counts.max() > 5000
 
 
+@pytest.mark.parametrize("nulls", [False, True])
+def test_sparse_product_heavy_join_ranks_bits_of_the_oracle(cuda, nulls):
+    """More than 512 output columns and Zipf-hot join keys (the hottest j
+    holds about 6,400 rows per side, five hold over 1,024): the heavy z's
+    run as dense phases between the light ones, and every sum must still be
+    the oracle's bit for bit, NULL factors (5 %) included, at any batching."""
+    print(f"sparse product heavy join ranks nulls={nulls}", flush=True)
+    rows, ids = 40_000, 4000
+
+    def factor(salt):
+        v = data.normal(9, salt, rows)
+        mask = (data.next_u64(9, salt + 10, rows) % np.uint64(20) == 0) if nulls else None
+        return pa.array(v, pa.float64(), mask=mask)
+    a = pa.table({"i": pa.array(data.hash64(data.next_u64(9, 1, rows) % np.uint64(1500), 0x11), pa.uint64()),
+                  "j": pa.array(data.hash64(data.zipf_ids(9, 2, rows, ids, 1.1), 0x6A), pa.uint64()),
+                  "v": factor(3)})
+    b = pa.table({"j": pa.array(data.hash64(data.zipf_ids(9, 4, rows, ids, 1.1), 0x6A), pa.uint64()),
+                  "k": pa.array(data.hash64(data.next_u64(9, 5, rows) % np.uint64(1500), 0x22), pa.uint64()),
+                  "v": factor(6)})
+    _, counts = np.unique(b.column(0).to_numpy(), return_counts=True)
+    assert np.sort(counts)[-5] >= 1024
+    f = _matmul_fold()
+    got = kernels.fold(f, a.to_batches(max_chunksize=8192), b.to_batches(max_chunksize=8192))
+    again = kernels.fold(f, a.to_batches(max_chunksize=65536), b.to_batches(max_chunksize=65536))
+    assert got.algorithm == again.algorithm == "discriminate"
+    assert_same(got.batches, again.batches)
+    algo, want = R.fold(f, a.to_batches(), b.to_batches(), None, R.schema(f, [a.schema, b.schema]))
+    assert algo == "discriminate"
+    assert_same(got.batches, want)
+
+
 def test_device_memory_short_is_out_of_memory_not_a_fault(cuda):
     """Work larger than the device's memory: the DenseBackend product and
     the attention forward's own scratch each ask hipMalloc for terabytes,
