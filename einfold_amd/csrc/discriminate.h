@@ -67,10 +67,13 @@ ef_status group_bounds(const uint32_t *a_start, const uint32_t *a_z, const uint3
 // Gustavson's product for operands with heavy join ranks: g.acc and
 // g.state hold one zeroed cell per (x, y) (nx * ny), heavy_host/heavy_dev
 // the heavy z ranks in ascending order (host and device copies),
-// heavy_rows their b bucket sizes, split 2 * n_heavy * nx words of
-// scratch.  Leaves each x's groups staged as gustavson does.
+// heavy_rows their b bucket sizes; scratch: split 2 * n_heavy * nx words,
+// runs one word per heavy bucket row (heavy z k's from run_off[k], host
+// and device copies of the offsets), n_runs n_heavy words.  Leaves each
+// x's groups staged as gustavson does.
 ef_status gustavson_heavy(const GustavsonArgs &g, const uint32_t *heavy_host, const uint32_t *heavy_dev,
-                          const uint32_t *heavy_rows, int n_heavy, uint32_t *split, hipStream_t s);
+                          const uint32_t *heavy_rows, int n_heavy, uint32_t *split, uint32_t *runs,
+                          const uint64_t *run_off_host, const uint64_t *run_off_dev, uint32_t *n_runs, hipStream_t s);
 ef_status gustavson(const GustavsonArgs &g, int blocks, hipStream_t s);
 ef_status widen_counts(const uint32_t *in, int64_t n, uint64_t *out, hipStream_t s);
 ef_status emit_groups(int64_t nx, const uint64_t *bound_off, const uint64_t *out_off, const uint32_t *g_count,

@@ -424,35 +424,110 @@ __global__ __launch_bounds__(256) void gustavson_light_kernel(GustavsonArgs g, c
     }
 }
 
-// One heavy z: blocks of 256 rows of its bucket, per x (blockIdx.x =
-// x * per_x + block): the thread on a run's first row sums the run against
-// the x's a rows at z.
+// The first row of each run of equal y in heavy z's b bucket, in order,
+// and their count (one workgroup of 1024 threads per heavy z).
+__global__ __launch_bounds__(1024) void heavy_runs_kernel(const uint32_t *b_start, const uint32_t *b_y,
+                                                          const uint32_t *heavy, const uint64_t *run_off,
+                                                          uint32_t *run_start, uint32_t *run_count) {
+    __shared__ uint32_t wave_n[16];
+    __shared__ uint32_t base;
+    const uint32_t h = heavy[blockIdx.x];
+    const uint32_t bs = b_start[h], be = b_start[h + 1];
+    uint32_t *out = run_start + run_off[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (uint32_t j0 = bs; j0 < be; j0 += 1024) {
+        const uint32_t j = j0 + tid;
+        const bool head = j < be && (j == bs || b_y[j - 1] != b_y[j]);
+        const uint64_t m = __ballot(head);
+        if (lane == 0) wave_n[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = base;
+        for (int w = 0; w < wv; w++) before += wave_n[w];
+        if (head) out[before + __popcll(m & (lane == 0 ? 0ULL : (~0ULL >> (64 - lane))))] = j;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < 16; w++) t += wave_n[w];
+            base += t;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) run_count[blockIdx.x] = base;
+}
+
+// One heavy z: a thread per run of equal y in its bucket (runs [r0, r0 +
+// 256) of block r0 / 256 in each group of HX x's: blockIdx.x = x group *
+// per_x + block) sums the run against each x's a rows at z.  Every cell's
+// word is loaded before any is stored, so the loads of the HX cells are in
+// flight together.
+constexpr int HX = 16;
+
 __global__ __launch_bounds__(256) void gustavson_heavy_kernel(GustavsonArgs g, uint32_t h, const uint32_t *lo_idx,
-                                                              const uint32_t *hi_idx, int64_t per_x) {
-    const int64_t x = (int64_t)blockIdx.x / per_x;
-    const uint32_t i0 = lo_idx[x], i1 = hi_idx[x];
-    if (i0 == i1) return;
-    const uint32_t bs = g.b_start[h], be = g.b_start[h + 1];
-    const int64_t j = (int64_t)bs + ((int64_t)blockIdx.x % per_x) * 256 + threadIdx.x;
-    if (j >= be) return;
+                                                              const uint32_t *hi_idx, const uint32_t *runs,
+                                                              const uint32_t *n_runs, int64_t per_x) {
+    const int64_t x0 = (int64_t)blockIdx.x / per_x * HX;
+    const int64_t r = ((int64_t)blockIdx.x % per_x) * 256 + threadIdx.x;
+    const uint32_t nr = *n_runs;
+    if (r >= nr) return;
+    const uint32_t be = g.b_start[h + 1];
+    const uint32_t j = runs[r], e = r + 1 < nr ? runs[r + 1] : be;
     const uint32_t y = g.b_y[j];
-    if (j != bs && g.b_y[j - 1] == y) return;   // not the head of its run
-    double *cell = g.acc + x * g.ny + y;
-    uint8_t *cst = g.state + x * g.ny + y;
-    double a = *cell;
-    uint8_t st = *cst | 1;
-    for (uint32_t i = i0; i < i1; i++) {
-        const bool av = g.a_valid[i] != 0;
-        const double p = g.a_value[i];
-        for (int64_t k = j; k < be && g.b_y[k] == y; k++) {
-            if (av && g.b_valid[k]) {
-                a = a + p * g.b_value[k];
-                st |= 2;
+    uint32_t i0[HX], i1[HX];
+    double a[HX];
+    uint8_t st[HX];
+#pragma unroll
+    for (int xi = 0; xi < HX; xi++) {
+        const int64_t x = x0 + xi < g.nx ? x0 + xi : g.nx - 1;
+        i0[xi] = lo_idx[x];
+        i1[xi] = x0 + xi < g.nx ? hi_idx[x] : i0[xi];
+    }
+#pragma unroll
+    for (int xi = 0; xi < HX; xi++) {
+        if (i0[xi] == i1[xi]) continue;
+        a[xi] = g.acc[(x0 + xi) * g.ny + y];
+        st[xi] = g.state[(x0 + xi) * g.ny + y];
+    }
+    // A short run's factors in registers (NULL as a zero flag), read once.
+    constexpr int RUN = 8;
+    const bool short_run = e - j <= (uint32_t)RUN;
+    double bv[RUN];
+    bool bok[RUN];
+#pragma unroll
+    for (int q = 0; q < RUN; q++) {
+        const uint32_t k = j + q < e ? j + q : j;
+        bv[q] = g.b_value[k];
+        bok[q] = j + q < e && g.b_valid[k] != 0;
+    }
+#pragma unroll
+    for (int xi = 0; xi < HX; xi++) {
+        if (i0[xi] == i1[xi]) continue;
+        double s = a[xi];
+        uint8_t t = st[xi] | 1;
+        for (uint32_t i = i0[xi]; i < i1[xi]; i++) {
+            const bool av = g.a_valid[i] != 0;
+            const double p = g.a_value[i];
+            if (short_run) {
+#pragma unroll
+                for (int q = 0; q < RUN; q++) {
+                    if (av && bok[q]) {
+                        s = s + p * bv[q];
+                        t |= 2;
+                    }
+                }
+            } else {
+                for (uint32_t k = j; k < e; k++) {
+                    if (av && g.b_valid[k]) {
+                        s = s + p * g.b_value[k];
+                        t |= 2;
+                    }
+                }
             }
         }
+        g.acc[(x0 + xi) * g.ny + y] = s;
+        g.state[(x0 + xi) * g.ny + y] = t;
     }
-    *cell = a;
-    *cst = st;
 }
 
 // Each x's groups from its cells' states, in y order, into its staged
@@ -748,23 +823,28 @@ ef_status gustavson(const GustavsonArgs &g, int blocks, hipStream_t s) {
 }
 
 ef_status gustavson_heavy(const GustavsonArgs &g, const uint32_t *heavy_host, const uint32_t *heavy_dev,
-                          const uint32_t *heavy_rows, int n_heavy, uint32_t *split, hipStream_t s) {
+                          const uint32_t *heavy_rows, int n_heavy, uint32_t *split, uint32_t *runs,
+                          const uint64_t *run_off_host, const uint64_t *run_off_dev, uint32_t *n_runs, hipStream_t s) {
     if (g.nx == 0) return EF_OK;
     const unsigned wave_blocks = (unsigned)((g.nx + 3) / 4 < 4096 ? (g.nx + 3) / 4 : 4096);
     EF_LAUNCH(heavy_split_kernel, g.nx, g.a_start, g.a_z, g.nx, heavy_dev, n_heavy, split);
+    hipLaunchKernelGGL(heavy_runs_kernel, dim3((unsigned)n_heavy), dim3(1024), 0, s, g.b_start, g.b_y, heavy_dev,
+                       run_off_dev, runs, n_runs);
+    const int64_t groups = (g.nx + HX - 1) / HX;
     for (int k = 0; k <= n_heavy; k++) {
         const uint32_t *lo = k == 0 ? nullptr : split + (2 * (int64_t)(k - 1) + 1) * g.nx;
         const uint32_t *hi = k == n_heavy ? nullptr : split + (2 * (int64_t)k) * g.nx;
         hipLaunchKernelGGL(gustavson_light_kernel, dim3(wave_blocks), dim3(256), 0, s, g, lo, hi);
         if (k == n_heavy) break;
-        const int64_t per_x = ((int64_t)heavy_rows[k] + 255) / 256;
+        const int64_t per_x = ((int64_t)heavy_rows[k] + 255) / 256;   // runs are at most the bucket's rows
         if (per_x == 0) continue;
-        if (per_x * g.nx > 0x7fffffffLL) {
+        if (per_x * groups > 0x7fffffffLL) {
             set_error("a heavy join rank's bucket is too large for one launch");
             return EF_INVALID;
         }
-        hipLaunchKernelGGL(gustavson_heavy_kernel, dim3((unsigned)(per_x * g.nx)), dim3(256), 0, s, g, heavy_host[k],
-                           split + (2 * (int64_t)k) * g.nx, split + (2 * (int64_t)k + 1) * g.nx, per_x);
+        hipLaunchKernelGGL(gustavson_heavy_kernel, dim3((unsigned)(per_x * groups)), dim3(256), 0, s, g, heavy_host[k],
+                           split + (2 * (int64_t)k) * g.nx, split + (2 * (int64_t)k + 1) * g.nx,
+                           runs + run_off_host[k], n_runs + k, per_x);
     }
     hipLaunchKernelGGL(collect_groups_kernel, dim3(wave_blocks), dim3(256), 0, s, g);
     EF_HIP(hipGetLastError());

@@ -1114,11 +1114,23 @@ ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Re
         EF_TRY(c.zeros(g.acc, (size_t)(nx * ny)));
         EF_TRY(c.zeros(g.state, (size_t)(nx * ny)));
         EF_TRY(c.end_zeros());
-        uint32_t *heavy_dev, *split;
+        std::vector<uint64_t> run_off(heavy.size());
+        uint64_t run_total = 0;
+        for (size_t k = 0; k < heavy.size(); k++) {
+            run_off[k] = run_total;
+            run_total += heavy_rows[k];
+        }
+        uint32_t *heavy_dev, *split, *runs, *n_runs;
+        uint64_t *run_off_dev;
         EF_TRY(c.alloc(heavy_dev, heavy.size()));
         EF_TRY(c.upload<uint32_t>(heavy_dev, heavy.data(), heavy.size()));
+        EF_TRY(c.alloc(run_off_dev, heavy.size()));
+        EF_TRY(c.upload<uint64_t>(run_off_dev, run_off.data(), heavy.size()));
         EF_TRY(c.alloc(split, 2 * heavy.size() * (size_t)nx));
-        EF_TRY(gustavson_heavy(g, heavy.data(), heavy_dev, heavy_rows.data(), (int)heavy.size(), split, c.stream));
+        EF_TRY(c.alloc(runs, (size_t)run_total));
+        EF_TRY(c.alloc(n_runs, heavy.size()));
+        EF_TRY(gustavson_heavy(g, heavy.data(), heavy_dev, heavy_rows.data(), (int)heavy.size(), split, runs,
+                               run_off.data(), run_off_dev, n_runs, c.stream));
     } else {
         EF_TRY(c.zeros(g.acc, (size_t)(slots * ny)));
         EF_TRY(c.zeros(g.state, (size_t)(slots * ny)));
