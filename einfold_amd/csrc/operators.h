@@ -157,10 +157,12 @@ struct Call {
         pending_zeros.clear();
         return EF_OK;
     }
+    // n T from host memory (pageable) to the device; the host buffer may
+    // be reused when this returns.
     template <class T> ef_status upload(T *dst, const void *src, size_t n) {
-        if (n) EF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream));
-        return EF_OK;
+        return upload_bytes(dst, src, n * sizeof(T));
     }
+    ef_status upload_bytes(void *dst, const void *src, size_t bytes);
     template <class T> ef_status download(void *dst, const T *src, size_t n) {
         if (n) EF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
         return EF_OK;
