@@ -274,6 +274,7 @@ struct Stager {
     char *ring = nullptr;
     hipEvent_t ev[STAGE_SLOTS] = {};
     bool pending[STAGE_SLOTS] = {};
+    int next = 0;   // the slot the next chunk takes: successive copies rotate through the ring
     bool ready = false, failed = false;
     // Workers: part q of the current job (src, dst, len) for q = 1 .. THREADS - 1.
     std::mutex qmu;
@@ -360,8 +361,9 @@ ef_status Call::upload_bytes(void *dst, const void *src, size_t bytes) {
         Stager &st = stager();
         std::lock_guard<std::mutex> lock(st.mu);
         if (st.start()) {
-            int k = 0;
-            for (size_t off = 0; off < bytes; off += STAGE_CHUNK, k = (k + 1) % STAGE_SLOTS) {
+            for (size_t off = 0; off < bytes; off += STAGE_CHUNK) {
+                const int k = st.next;
+                st.next = (k + 1) % STAGE_SLOTS;
                 const size_t n = bytes - off < STAGE_CHUNK ? bytes - off : STAGE_CHUNK;
                 char *slot = st.ring + (size_t)k * STAGE_CHUNK;
                 if (st.pending[k]) {
