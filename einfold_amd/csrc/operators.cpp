@@ -255,9 +255,68 @@ void chunk_drop() {
     }
 }
 
+// A small pool of host threads for the library's own host work: run(f)
+// calls f(0) .. f(POOL_THREADS - 1) at once, f(0) on the caller, and
+// returns when all are done; one job at a time.  The workers live until the
+// process ends.
+namespace {
+constexpr int POOL_THREADS = 4;
+
+struct WorkPool {
+    std::mutex run_mu;   // one job at a time
+    std::mutex qmu;
+    std::condition_variable go, done;
+    uint64_t generation = 0;
+    int finished = 0;
+    const std::function<void(int)> *job = nullptr;
+    bool started = false;
+
+    void worker(int q) {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)> *f;
+            {
+                std::unique_lock<std::mutex> lock(qmu);
+                go.wait(lock, [&] { return generation != seen; });
+                seen = generation;
+                f = job;
+            }
+            (*f)(q);
+            {
+                std::lock_guard<std::mutex> lock(qmu);
+                finished++;
+            }
+            done.notify_one();
+        }
+    }
+    void run(const std::function<void(int)> &f) {
+        std::lock_guard<std::mutex> once(run_mu);
+        if (!started) {
+            for (int q = 1; q < POOL_THREADS; q++) std::thread([this, q] { worker(q); }).detach();
+            started = true;
+        }
+        {
+            std::lock_guard<std::mutex> lock(qmu);
+            job = &f;
+            finished = 0;
+            generation++;
+        }
+        go.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> lock(qmu);
+        done.wait(lock, [&] { return finished == POOL_THREADS - 1; });
+    }
+};
+
+WorkPool &work_pool() {
+    static WorkPool *p = new WorkPool;   // lives until the process ends, its workers too
+    return *p;
+}
+}  // namespace
+
 // Large host-to-device copies from pageable memory: HIP stages those
 // through its own pinned buffers with one host thread (about 30 GB/s
-// here).  Copied instead by STAGE_THREADS threads into a ring of pinned
+// here).  Copied instead by the work pool's threads into a ring of pinned
 // chunks, each sent by DMA as soon as it is full while the next fills,
 // they run at close to the link's 57 GB/s.  One staged copy at a time in
 // the process; a chunk is refilled only after its DMA has completed (its
@@ -267,7 +326,6 @@ namespace {
 constexpr size_t STAGE_FROM = (size_t)4 << 20;
 constexpr size_t STAGE_CHUNK = (size_t)16 << 20;
 constexpr int STAGE_SLOTS = 4;
-constexpr int STAGE_THREADS = 4;
 
 struct Stager {
     std::mutex mu;   // one staged copy at a time
@@ -276,43 +334,8 @@ struct Stager {
     bool pending[STAGE_SLOTS] = {};
     int next = 0;   // the slot the next chunk takes: successive copies rotate through the ring
     bool ready = false, failed = false;
-    // Workers: part q of the current job (src, dst, len) for q = 1 .. THREADS - 1.
-    std::mutex qmu;
-    std::condition_variable go, done;
-    uint64_t generation = 0;
-    int finished = 0;
-    const char *src = nullptr;
-    char *dst = nullptr;
-    size_t len = 0;
 
-    static void part(const char *s, char *d, size_t n, int q) {
-        const size_t per = (n + STAGE_THREADS - 1) / STAGE_THREADS;
-        const size_t a = per * (size_t)q, b = a + per < n ? a + per : n;
-        if (a < b) std::memcpy(d + a, s + a, b - a);
-    }
-    void worker(int q) {
-        uint64_t seen = 0;
-        for (;;) {
-            const char *s;
-            char *d;
-            size_t n;
-            {
-                std::unique_lock<std::mutex> lock(qmu);
-                go.wait(lock, [&] { return generation != seen; });
-                seen = generation;
-                s = src;
-                d = dst;
-                n = len;
-            }
-            part(s, d, n, q);
-            {
-                std::lock_guard<std::mutex> lock(qmu);
-                finished++;
-            }
-            done.notify_one();
-        }
-    }
-    // Whether the ring and the workers exist (made at the first staged copy).
+    // Whether the ring exists (made at the first staged copy).
     bool start() {
         if (ready || failed) return ready;
         void *p = nullptr;
@@ -329,28 +352,21 @@ struct Stager {
                 return false;
             }
         }
-        for (int q = 1; q < STAGE_THREADS; q++) std::thread([this, q] { worker(q); }).detach();
         ready = true;
         return true;
     }
-    void copy(const char *s, char *d, size_t n) {
-        {
-            std::lock_guard<std::mutex> lock(qmu);
-            src = s;
-            dst = d;
-            len = n;
-            finished = 0;
-            generation++;
-        }
-        go.notify_all();
-        part(s, d, n, 0);
-        std::unique_lock<std::mutex> lock(qmu);
-        done.wait(lock, [&] { return finished == STAGE_THREADS - 1; });
+    static void copy(const char *s, char *d, size_t n) {
+        const std::function<void(int)> f = [=](int q) {
+            const size_t per = (n + POOL_THREADS - 1) / POOL_THREADS;
+            const size_t a = per * (size_t)q, b = a + per < n ? a + per : n;
+            if (a < b) std::memcpy(d + a, s + a, b - a);
+        };
+        work_pool().run(f);
     }
 };
 
 Stager &stager() {
-    static Stager *s = new Stager;   // lives until the process ends, its workers too
+    static Stager *s = new Stager;   // lives until the process ends
     return *s;
 }
 }  // namespace
@@ -1943,10 +1959,62 @@ ef_status prepare(const FoldModel &m, const ef_relation *a, const ef_relation *b
 // chain.rs:57: the largest bound on a partial sum's magnitude taken, 2^1000.
 constexpr double CHAIN_LARGEST_BOUND = 1.0715086071862673e301;
 
+// Runs fn(lo, hi, part) over [0, n) in `parts` (1 or POOL_THREADS) parts,
+// on the work pool when more than one; the caller combines the parts'
+// results in part order.
+template <class F> void parallel_parts(int64_t n, int parts, F fn) {
+    if (parts <= 1) {
+        fn((int64_t)0, n, 0);
+        return;
+    }
+    const int64_t per = (n + parts - 1) / parts;
+    const std::function<void(int)> f = [&](int q) {
+        const int64_t lo = std::min(per * q, n), hi = std::min(lo + per, n);
+        fn(lo, hi, q);
+    };
+    work_pool().run(f);
+}
+
+// One part below 2^18 elements, else the pool's.
+int parts_for(int64_t n) { return n >= ((int64_t)1 << 18) ? POOL_THREADS : 1; }
+
+// The largest magnitude of the non-NULL values of a Float64 column as the
+// largest of their magnitudes' bit patterns (for finite doubles the same
+// order), which is at least 0x7ff0... exactly when one is inf or NaN.
+uint64_t largest_magnitude_bits(const ColumnView &v) {
+    const uint64_t *x = reinterpret_cast<const uint64_t *>(v.values);
+    constexpr uint64_t ABS = 0x7fffffffffffffffULL;
+    const int parts = parts_for(v.length);
+    uint64_t part_max[POOL_THREADS] = {};
+    parallel_parts(v.length, parts, [&](int64_t lo, int64_t hi, int q) {
+        uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+        int64_t i = lo;
+        if (v.null_count == 0) {
+            for (; i + 4 <= hi; i += 4) {
+                const uint64_t a = x[i] & ABS, b = x[i + 1] & ABS, c = x[i + 2] & ABS, d = x[i + 3] & ABS;
+                m0 = a > m0 ? a : m0;
+                m1 = b > m1 ? b : m1;
+                m2 = c > m2 ? c : m2;
+                m3 = d > m3 ? d : m3;
+            }
+        }
+        for (; i < hi; i++) {
+            const uint64_t a = v.valid(i) ? x[i] & ABS : 0;
+            m0 = a > m0 ? a : m0;
+        }
+        m0 = std::max(std::max(m0, m1), std::max(m2, m3));
+        part_max[q] = m0;
+    });
+    uint64_t m = 0;
+    for (int q = 0; q < parts; q++) m = std::max(m, part_max[q]);
+    return m;
+}
+
 // chain.rs:155-181: every non-NULL factor is a finite Float64, and the
 // product over the operands of their largest magnitude times their rows,
 // which bounds every partial sum in either order, is at most 2^1000.
 bool chain_guarded(const FoldModel &m, const std::vector<const Relation *> &ops) {
+    constexpr uint64_t INF_BITS = 0x7ff0000000000000ULL;
     double bound = 1.0;
     for (size_t o = 0; o < ops.size(); o++) {
         const int c = m.operands[o].factor;
@@ -1955,27 +2023,11 @@ bool chain_guarded(const FoldModel &m, const std::vector<const Relation *> &ops)
         double largest = 0.0;
         int64_t rows = 0;
         for (size_t b = 0; b < r.batches.size(); b++) {
-            const ColumnView &v = r.batches[b][c];
             rows += r.batch_rows[b];
-            const double *x = reinterpret_cast<const double *>(v.values);
-            // Branch-free passes (the compiler vectorizes them): |x| <= the
-            // largest finite double fails for inf and NaN alike.
-            bool finite = true;
-            double big = 0.0;
-            if (v.null_count == 0) {
-                for (int64_t i = 0; i < v.length; i++) {
-                    const double a = std::fabs(x[i]);
-                    finite &= a <= std::numeric_limits<double>::max();
-                    big = a > big ? a : big;
-                }
-            } else {
-                for (int64_t i = 0; i < v.length; i++) {
-                    const double a = v.valid(i) ? std::fabs(x[i]) : 0.0;
-                    finite &= a <= std::numeric_limits<double>::max();
-                    big = a > big ? a : big;
-                }
-            }
-            if (!finite) return false;
+            const uint64_t bits = largest_magnitude_bits(r.batches[b][c]);
+            if (bits >= INF_BITS) return false;
+            double big;
+            std::memcpy(&big, &bits, 8);
             largest = std::max(largest, big);
         }
         bound *= largest * (double)rows;
@@ -1983,38 +2035,59 @@ bool chain_guarded(const FoldModel &m, const std::vector<const Relation *> &ops)
     return bound <= CHAIN_LARGEST_BOUND;
 }
 
-// The largest key of a column of integer type T, and whether one is
-// negative (unsigned 64-bit keys of 2^63 and above count as negative: too
-// large for i64), branch-free.
-template <class T> void key_range(const T *k, int64_t n, int64_t &hi, bool &negative) {
-    int64_t h = -1;
-    bool neg = false;
-    for (int64_t i = 0; i < n; i++) {
-        const int64_t x = (int64_t)k[i];
-        neg |= x < 0;
-        h = x > h ? x : h;
+// The smallest and largest key of a column of integer type T (as i64; an
+// unsigned 64-bit key of 2^63 or more comes out negative: too large for
+// i64), over parallel parts with four accumulators each.
+template <class T> void key_range(const T *k, int64_t n, int64_t &lo_out, int64_t &hi_out) {
+    const int parts = parts_for(n);
+    int64_t part_lo[POOL_THREADS], part_hi[POOL_THREADS];
+    parallel_parts(n, parts, [&](int64_t lo, int64_t hi, int q) {
+        int64_t l0 = INT64_MAX, l1 = INT64_MAX, h0 = INT64_MIN, h1 = INT64_MIN;
+        int64_t i = lo;
+        for (; i + 2 <= hi; i += 2) {
+            const int64_t a = (int64_t)k[i], b = (int64_t)k[i + 1];
+            l0 = a < l0 ? a : l0;
+            h0 = a > h0 ? a : h0;
+            l1 = b < l1 ? b : l1;
+            h1 = b > h1 ? b : h1;
+        }
+        for (; i < hi; i++) {
+            const int64_t a = (int64_t)k[i];
+            l0 = a < l0 ? a : l0;
+            h0 = a > h0 ? a : h0;
+        }
+        part_lo[q] = std::min(l0, l1);
+        part_hi[q] = std::max(h0, h1);
+    });
+    int64_t l = INT64_MAX, h = INT64_MIN;
+    for (int q = 0; q < parts; q++) {
+        l = std::min(l, part_lo[q]);
+        h = std::max(h, part_hi[q]);
     }
-    hi = h;
-    negative = neg;
+    lo_out = l;
+    hi_out = h;
 }
 
 // dense.rs:97-111 for one batch's key column: integers without NULLs, each
 // in [0, 2^63) and below i64::MAX; the extent is the largest plus one.
 bool chain_key_extent(const ColumnView &v, int64_t &extent) {
     if (!is_integer(v.type) || v.null_count > 0) return false;
-    int64_t hi = -1;
-    bool negative = false;
-    switch (v.type) {
-    case Type::Int8: key_range(reinterpret_cast<const int8_t *>(v.values), v.length, hi, negative); break;
-    case Type::UInt8: key_range(reinterpret_cast<const uint8_t *>(v.values), v.length, hi, negative); break;
-    case Type::Int16: key_range(reinterpret_cast<const int16_t *>(v.values), v.length, hi, negative); break;
-    case Type::UInt16: key_range(reinterpret_cast<const uint16_t *>(v.values), v.length, hi, negative); break;
-    case Type::Int32: key_range(reinterpret_cast<const int32_t *>(v.values), v.length, hi, negative); break;
-    case Type::UInt32: key_range(reinterpret_cast<const uint32_t *>(v.values), v.length, hi, negative); break;
-    case Type::Int64: key_range(reinterpret_cast<const int64_t *>(v.values), v.length, hi, negative); break;
-    default: key_range(reinterpret_cast<const uint64_t *>(v.values), v.length, hi, negative); break;
+    if (v.length == 0) {
+        extent = 0;
+        return true;
     }
-    if (negative || hi == std::numeric_limits<int64_t>::max()) return false;
+    int64_t lo = 0, hi = -1;
+    switch (v.type) {
+    case Type::Int8: key_range(reinterpret_cast<const int8_t *>(v.values), v.length, lo, hi); break;
+    case Type::UInt8: key_range(reinterpret_cast<const uint8_t *>(v.values), v.length, lo, hi); break;
+    case Type::Int16: key_range(reinterpret_cast<const int16_t *>(v.values), v.length, lo, hi); break;
+    case Type::UInt16: key_range(reinterpret_cast<const uint16_t *>(v.values), v.length, lo, hi); break;
+    case Type::Int32: key_range(reinterpret_cast<const int32_t *>(v.values), v.length, lo, hi); break;
+    case Type::UInt32: key_range(reinterpret_cast<const uint32_t *>(v.values), v.length, lo, hi); break;
+    case Type::Int64: key_range(reinterpret_cast<const int64_t *>(v.values), v.length, lo, hi); break;
+    default: key_range(reinterpret_cast<const uint64_t *>(v.values), v.length, lo, hi); break;
+    }
+    if (lo < 0 || hi == std::numeric_limits<int64_t>::max()) return false;
     extent = hi + 1;
     return true;
 }
