@@ -900,7 +900,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
             // The tile's queries in two 32-query halves (hc): S = Q K^T and
             // dP = dO V^T for its two 16-query blocks, then P and dS, then
             // dV += P^T dO and dK += dS^T Q over the half.
-#pragma unroll
+#pragma unroll 1
             for (int hc = 0; hc < 2; hc++) {
                 AccV s[2][2], dp[2][2];   // [query block][key block]
 #pragma unroll
@@ -1144,7 +1144,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
             if (PF && j0 + BN < last) EF_FETCH(j0 + BN);
             // The tile's keys in two 32-key halves: S^T = K Q^T and
             // dP^T = V dO^T, then dS^T, then dQ^T += K^T dS^T.
-#pragma unroll
+#pragma unroll 1
             for (int hc = 0; hc < 2; hc++) {
                 AccV s[2][2], dp[2][2];   // [key block][query block]
 #pragma unroll
