@@ -174,7 +174,7 @@ template <int P, int BM, int BN, int WAVES, int KSTEP> struct GemmOcc {
 template <int P, int BM, int BN, int WGM, int WGN, int KSTEP>
 __global__ __launch_bounds__(64 * WGM * WGN)
 __attribute__((amdgpu_waves_per_eu(GemmOcc<P, BM, BN, WGM * WGN, KSTEP>::per_simd)))
-void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, int n_items, const typename Elem<P>::T *ops) {
+void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, const int *wg_off, const typename Elem<P>::T *ops) {
     typedef typename Elem<P>::T E;
     typedef typename Acc<P>::type AccV;
     constexpr int NT = 64 * WGM * WGN;
@@ -202,9 +202,9 @@ void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, int n_items, 
         crow_[i] = (kq % (rows * per_row)) / per_row;
         ccol[i] = (kq % per_row) * CE;
     }
-    // Items are sorted heaviest first; workgroup g takes g, g + grid, ...
-    // (the hardware starts workgroups in index order as CUs free up).
-    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    // Workgroup g computes items wg_off[g] .. wg_off[g + 1] - 1 (the host
+    // balanced the CUs' shares).
+    for (int it = wg_off[blockIdx.x]; it < wg_off[blockIdx.x + 1]; it++) {
         const Item item = items[it];
         const GemmDesc d = descs[item.problem];
         const int64_t m0 = (int64_t)item.tm * BM, n0 = (int64_t)item.tn * BN;
@@ -338,7 +338,7 @@ template <int> struct Parity {};
 template <int P, int BM, int BN, int WGM, int WGN, int KSTEP>
 __global__ __launch_bounds__(64 * WGM * WGN)
 __attribute__((amdgpu_waves_per_eu(GemmOcc<P, BM, BN, WGM * WGN, KSTEP>::per_simd)))
-void grouped_gemm_dma_kernel(const GemmDesc *descs, const Item *items, int n_items, const typename Elem<P>::T *ops) {
+void grouped_gemm_dma_kernel(const GemmDesc *descs, const Item *items, const int *wg_off, const typename Elem<P>::T *ops) {
     typedef typename Elem<P>::T E;
     typedef typename Acc<P>::type AccV;
     constexpr int NW = WGM * WGN;
@@ -354,7 +354,7 @@ void grouped_gemm_dma_kernel(const GemmDesc *descs, const Item *items, int n_ite
     __shared__ __attribute__((aligned(1024))) char buf1[BUFB];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
     const int wr = w / WGN, wc = w % WGN;
-    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    for (int it = wg_off[blockIdx.x]; it < wg_off[blockIdx.x + 1]; it++) {
         const Item item = items[it];
         const GemmDesc d = descs[item.problem];
         const int64_t m0 = (int64_t)item.tm * BM, n0 = (int64_t)item.tn * BN;
@@ -452,6 +452,7 @@ struct GroupedPlan {
     GemmDesc *descs = nullptr;
     PrepTask *tasks = nullptr;
     Item *items = nullptr;
+    int *wg_off = nullptr;            // grid + 1: workgroup g's items are wg_off[g] .. wg_off[g + 1] - 1
     void *ops = nullptr;
     int n_tasks = 0, n_items = 0, grid = 0;
     hipGraph_t graph = nullptr;
@@ -470,6 +471,34 @@ int cus() {
                                                hipSuccess && n > 0)
         return n;
     return 256;
+}
+
+// Workgroups of the plan's GEMM kernel that one CU holds at once (by the
+// runtime's occupancy calculator; 1 if it cannot tell).
+template <class K> int occupancy(K kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n;
+}
+
+template <int P> int gemm_slots_of(int config) {
+    if (config == 0) return occupancy(grouped_gemm_kernel<P, 128, 128, 2, 2, 64>, 256);
+    if (config == 1) return occupancy(grouped_gemm_kernel<P, 64, 64, 2, 2, 64>, 256);
+    if constexpr (P == PREC_BF16) return occupancy(grouped_gemm_dma_kernel<P, 256, 256, 2, 4, 64>, 512);
+    if constexpr (P == PREC_BF16X3) return occupancy(grouped_gemm_kernel<P, 256, 128, 4, 2, 32>, 512);
+    return 1;
+}
+
+int gemm_slots(const GroupedPlan &p) {
+    switch (p.precision) {
+    case EF_PRECISION_FP32: return gemm_slots_of<PREC_F32>(p.config);
+    case EF_PRECISION_BF16X3: return gemm_slots_of<PREC_BF16X3>(p.config);
+    case EF_PRECISION_BF16: return gemm_slots_of<PREC_BF16>(p.config);
+    default: return 1;
+    }
 }
 
 ef_status build(GroupedPlan &p, hipStream_t s) {
@@ -504,25 +533,71 @@ ef_status build(GroupedPlan &p, hipStream_t s) {
     }
     // Largest k first; ties in problem and tile order (deterministic list).
     std::stable_sort(items.begin(), items.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    // Workgroups: as many per CU as fit at once (SLOTS; the hardware puts
+    // workgroups g, g + C, g + 2 C, ... on one CU of C), at most one per
+    // item.  Each item goes,
+    // heaviest first, to the CU with the least k steps so far (with a free
+    // workgroup while items fit one per workgroup), then to that CU's
+    // least loaded workgroup: every CU gets about the same work, where the
+    // round-robin order gave some CUs the heaviest item of every round.
+    const int SLOTS = gemm_slots(p);
+    const int C = cus();
+    const int n = (int)items.size();
+    const int grid = n <= C ? n : SLOTS * C;
+    std::vector<std::vector<int>> per_wg((size_t)std::max(grid, 1));
+    if (n <= C) {
+        for (int i = 0; i < n; i++) per_wg[i].push_back(i);
+    } else {
+        const bool one_each = n <= SLOTS * C;
+        std::vector<int64_t> cu_load(C, 0), wg_load(grid, 0);
+        std::vector<int> cu_used(C, 0);
+        for (int i = 0; i < n; i++) {
+            int best = -1;
+            for (int c = 0; c < C; c++) {
+                if (one_each && cu_used[c] >= SLOTS) continue;
+                if (best < 0 || cu_load[c] < cu_load[best]) best = c;
+            }
+            int wg = -1;
+            for (int j = 0; j < SLOTS; j++) {
+                const int g = best + j * C;
+                if (g >= grid) break;
+                if (one_each && !per_wg[g].empty()) continue;
+                if (wg < 0 || wg_load[g] < wg_load[wg]) wg = g;
+            }
+            per_wg[wg].push_back(i);
+            cu_load[best] += items[i].first;
+            wg_load[wg] += items[i].first;
+            cu_used[best]++;
+        }
+    }
+    std::vector<int> wg_off(grid + 1, 0);
+    std::vector<std::pair<int64_t, Item>> ordered;
+    for (int g = 0; g < grid; g++) {
+        wg_off[g] = (int)ordered.size();
+        for (int i : per_wg[g]) ordered.push_back(items[i]);
+    }
+    wg_off[grid] = (int)ordered.size();
+    items.swap(ordered);
     p.n_tasks = (int)tasks.size();
     p.n_items = (int)items.size();
     const size_t bytes = up256(descs.size() * sizeof(GemmDesc)) + up256(tasks.size() * sizeof(PrepTask)) +
-                         up256(items.size() * sizeof(Item)) + up256((size_t)off * esz);
+                         up256(items.size() * sizeof(Item)) + up256(wg_off.size() * sizeof(int)) +
+                         up256((size_t)off * esz);
     EF_TRY(p.mem.alloc(bytes));
     char *at = static_cast<char *>(p.mem.ptr);
     p.descs = reinterpret_cast<GemmDesc *>(at); at += up256(descs.size() * sizeof(GemmDesc));
     p.tasks = reinterpret_cast<PrepTask *>(at); at += up256(tasks.size() * sizeof(PrepTask));
     p.items = reinterpret_cast<Item *>(at); at += up256(items.size() * sizeof(Item));
+    p.wg_off = reinterpret_cast<int *>(at); at += up256(wg_off.size() * sizeof(int));
     p.ops = at;
     std::vector<Item> flat;
     for (auto &x : items) flat.push_back(x.second);
     if (!descs.empty()) EF_HIP(hipMemcpyAsync(p.descs, descs.data(), descs.size() * sizeof(GemmDesc), hipMemcpyHostToDevice, s));
     if (!tasks.empty()) EF_HIP(hipMemcpyAsync(p.tasks, tasks.data(), tasks.size() * sizeof(PrepTask), hipMemcpyHostToDevice, s));
     if (!flat.empty()) EF_HIP(hipMemcpyAsync(p.items, flat.data(), flat.size() * sizeof(Item), hipMemcpyHostToDevice, s));
+    EF_HIP(hipMemcpyAsync(p.wg_off, wg_off.data(), wg_off.size() * sizeof(int), hipMemcpyHostToDevice, s));
     EF_HIP(hipStreamSynchronize(s));
-    // Up to four workgroups per CU where registers and LDS allow; the rest
-    // start when a slot frees and find the work list empty.
-    p.grid = std::min(p.n_items, 4 * cus());
+    p.grid = grid;
     return EF_OK;
 }
 
@@ -533,16 +608,16 @@ ef_status launch_steps(const GroupedPlan &p, hipStream_t s) {
     hipLaunchKernelGGL(grouped_prep_kernel<P>, dim3((unsigned)p.n_tasks), dim3(256), 0, s, p.descs, p.tasks, (E *)p.ops);
     if (p.config == 0) {
         hipLaunchKernelGGL((grouped_gemm_kernel<P, 128, 128, 2, 2, 64>), dim3((unsigned)p.grid), dim3(256), 0, s,
-                           p.descs, p.items, p.n_items, (const E *)p.ops);
+                           p.descs, p.items, p.wg_off, (const E *)p.ops);
     } else if (p.config == 1) {
         hipLaunchKernelGGL((grouped_gemm_kernel<P, 64, 64, 2, 2, 64>), dim3((unsigned)p.grid), dim3(256), 0, s,
-                           p.descs, p.items, p.n_items, (const E *)p.ops);
+                           p.descs, p.items, p.wg_off, (const E *)p.ops);
     } else if constexpr (P == PREC_BF16) {
         hipLaunchKernelGGL((grouped_gemm_dma_kernel<P, 256, 256, 2, 4, 64>), dim3((unsigned)p.grid), dim3(512), 0, s,
-                           p.descs, p.items, p.n_items, (const E *)p.ops);
+                           p.descs, p.items, p.wg_off, (const E *)p.ops);
     } else if constexpr (P == PREC_BF16X3) {
         hipLaunchKernelGGL((grouped_gemm_kernel<P, 256, 128, 4, 2, 32>), dim3((unsigned)p.grid), dim3(512), 0, s,
-                           p.descs, p.items, p.n_items, (const E *)p.ops);
+                           p.descs, p.items, p.wg_off, (const E *)p.ops);
     } else {
         set_error("no large tile shape for FP32 products");
         return EF_INVALID;
