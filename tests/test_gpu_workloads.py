@@ -190,6 +190,68 @@ def test_sparse_product_heavy_join_ranks_bits_of_the_oracle(cuda, nulls):
     assert_same(got.batches, want)
 
 
+@pytest.mark.parametrize("missing", [None, 0.0])
+def test_sparse_product_heavy_join_ranks_with_a_domain(cuda, missing):
+    """The heavy-rank phases under a domain (discriminate.rs:611-648): each
+    domain row (keys drawn from the operands', some unseen, some NULL) gets
+    its group's sum or the default, bit for bit the oracle's."""
+    print(f"sparse product heavy join ranks with a domain missing={missing}", flush=True)
+    rows, ids = 40_000, 4000
+    a = pa.table({"i": pa.array(data.hash64(data.next_u64(10, 1, rows) % np.uint64(1500), 0x11), pa.uint64()),
+                  "j": pa.array(data.hash64(data.zipf_ids(10, 2, rows, ids, 1.1), 0x6A), pa.uint64()),
+                  "v": pa.array(data.normal(10, 3, rows), pa.float64())})
+    b = pa.table({"j": pa.array(data.hash64(data.zipf_ids(10, 4, rows, ids, 1.1), 0x6A), pa.uint64()),
+                  "k": pa.array(data.hash64(data.next_u64(10, 5, rows) % np.uint64(1500), 0x22), pa.uint64()),
+                  "v": pa.array(data.normal(10, 6, rows), pa.float64())})
+    nd = 20_000
+    pick = lambda col, salt: col.to_numpy()[data.next_u64(10, salt, nd) % np.uint64(rows)]
+    x = pick(a.column(0), 7)
+    y = pick(b.column(1), 8)
+    unseen = data.next_u64(10, 9, nd) % np.uint64(10) == 0
+    x = np.where(unseen, data.next_u64(10, 11, nd), x)
+    nulls = data.next_u64(10, 12, nd) % np.uint64(50) == 0
+    dom = pa.table({"x": pa.array(x, pa.uint64(), mask=nulls), "y": pa.array(y, pa.uint64()),
+                    "id": pa.array(np.arange(nd), pa.int64())})
+    f = ir.Fold([EQ] * 3, [ir.Operand((ir.Key(0, 0), ir.Key(1, 1)), 2),
+                           ir.Operand((ir.Key(1, 0), ir.Key(2, 1)), 2)],
+                [ir.Output(0, ir.DomainKey(0, EQ)), ir.Output(2, ir.DomainKey(1, EQ))],
+                domain=ir.Domain([2, 1], missing))
+    got = kernels.fold(f, a, b, dom)
+    algo, want = R.fold(f, a.to_batches(), b.to_batches(), dom.to_batches(),
+                        R.schema(f, [a.schema, b.schema], dom.schema))
+    assert algo == got.algorithm == "discriminate"
+    assert_same(got.batches, want)
+
+
+def test_concurrent_calls_from_two_threads(cuda):
+    """Two host threads calling the operators at once (the library's
+    staging ring, host work pool, pinned buffer pool and device chunk cache
+    are shared): each result equals the same call made alone."""
+    import threading
+    print("concurrent calls from two threads", flush=True)
+    a1, b1 = data.grid_matmul(1024, seed=3)      # 1M rows a side: columns above the staging threshold
+    a2, b2 = data.grid_matmul(1024, seed=4)
+    f = _matmul_fold()
+    alone = [kernels.fold_dense(f, a1, b1), kernels.fold_dense(f, a2, b2)]
+    results = [None, None]
+    errors = []
+
+    def run(i, a, b):
+        try:
+            for _ in range(3):
+                results[i] = kernels.fold_dense(f, a, b)
+        except Exception as e:   # reported below
+            errors.append(e)
+    threads = [threading.Thread(target=run, args=(0, a1, b1)), threading.Thread(target=run, args=(1, a2, b2))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    assert_same(results[0], alone[0])
+    assert_same(results[1], alone[1])
+
+
 def test_device_memory_short_is_out_of_memory_not_a_fault(cuda):
     """Work larger than the device's memory: the DenseBackend product and
     the attention forward's own scratch each ask hipMalloc for terabytes,
