@@ -26,6 +26,12 @@ FLAGS = [
 ]
 
 
+# Per-source flags.  attention_bwd.hip (the attention backward kernels):
+# MFMA accumulators in VGPRs by default, so that the pipelined kernels copy
+# fewer values between AGPRs and VGPRs around their softmax.
+SOURCE_FLAGS = {"attention_bwd.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
                   if f.endswith((".hip", ".cpp")))
@@ -33,7 +39,7 @@ def sources():
 
 def _compile(src: str) -> str:
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -23,11 +23,23 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "common.h"
 #include "mfma.h"
 
+// Compiled a second time as attention_bwd.hip with EF_ATTN_BWD_TU set: that
+// unit holds the backward kernels' instances (built with other code
+// generation options) and the launcher below, this one everything else.
+#ifndef EF_ATTN_BWD_TU
+#define EF_ATTN_BWD_TU 0
+#endif
+
 namespace ef {
+
+// Launches the dK/dV (kernel 0) or dQ (kernel 1) backward kernel for
+// precision P and head size D with the BwdArgs at args.
+void attn_bwd_launch(int kernel, int precision, int d, const void *args, unsigned grid, hipStream_t s);
 
 namespace {
 
@@ -254,7 +266,23 @@ template <class E, int PLN, int R, int C> struct TileIO {
     static constexpr int CE = 16 / sizeof(E);
     static constexpr int PER_ROW = C / CE, PER_PLANE = R * PER_ROW, TOTAL = PLN * PER_PLANE;
     static constexpr int N = (TOTAL + THREADS - 1) / THREADS;
+    // Chunk i's source splits into a part common to the workgroup (plane
+    // and row block of chunk i) and this thread's 32-bit offset, so that
+    // each load is a scalar base plus one register.
+    static constexpr bool SPLIT = PER_PLANE % THREADS == 0 && THREADS % PER_ROW == 0;
     __device__ static __forceinline__ void load(u32x4 (&buf)[N], const E *g, int64_t gstride, int64_t gplane) {
+        if constexpr (SPLIT) {
+            const unsigned lane_off = (unsigned)(((int)threadIdx.x / PER_ROW) * gstride +
+                                                 ((int)threadIdx.x % PER_ROW) * CE) * (unsigned)sizeof(E);
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                constexpr int BLOCKS = PER_PLANE / THREADS;
+                const int p = i / BLOCKS, rows = (i % BLOCKS) * (THREADS / PER_ROW);
+                const char *base = reinterpret_cast<const char *>(g + p * gplane + rows * gstride);
+                buf[i] = *reinterpret_cast<const u32x4 *>(base + (uint64_t)lane_off);
+            }
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const int k = (int)threadIdx.x + THREADS * i;
@@ -775,7 +803,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
     __shared__ __attribute__((aligned(16))) E dotr[PL * TPLANE];
     typedef typename Stat<P>::T St;
     __shared__ St sm[BN], sd[BN];   // per query: m log2(e) + log2(l), and delta
-    __shared__ int64_t slo[BN], shi[BN];
+    __shared__ int slo[BN], shi[BN];   // spans in 32 bits (the host keeps tq, tk < 2^31)
     __shared__ int slot;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, c = lane & 15;
@@ -806,11 +834,11 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         const St *RS = static_cast<const St *>(a.row_sum) + b * a.tq;
         const St *DL = static_cast<const St *>(a.delta) + b * a.tq;
 
-        int64_t kj[2];
+        int kj[2];
         Frag<P> kf[2][KC], vf[2][KC];
 #pragma unroll
         for (int kb = 0; kb < 2; kb++) {
-            kj[kb] = k0 + w * 32 + kb * 16 + c;
+            kj[kb] = (int)k0 + w * 32 + kb * 16 + c;
 #pragma unroll
             for (int kc = 0; kc < KC; kc++) {
                 In t1[8], t2[8];
@@ -870,6 +898,196 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         RowT::load(pd, DR + q0_ * D, D, rplane);                                             \
         ColT::load(pdt, DT + q0_, a.tqp, tplane);                                            \
     } while (0)
+        // FP64: each half's stages one after another, the mask tested
+        // unless the tile is full.
+        auto dkdv_tile_f64 = [&](bool full) __attribute__((always_inline)) {
+#pragma unroll 1
+                for (int hc = 0; hc < 2; hc++) {
+                    AccV s1[2][2], dp1[2][2];   // [query block][key block]
+#pragma unroll
+                    for (int qq = 0; qq < 2; qq++)
+#pragma unroll
+                        for (int kb = 0; kb < 2; kb++) { s1[qq][kb] = AccV{0, 0, 0, 0}; dp1[qq][kb] = AccV{0, 0, 0, 0}; }
+#pragma unroll
+                    for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+                        for (int qq = 0; qq < 2; qq++) {
+                            const int qb = 2 * hc + qq;
+                            Frag<P> af, df;
+                            tile_load8<P>(af, qs, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
+                            tile_load8<P>(df, dos, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
+#pragma unroll
+                            for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                                for (int kb = 0; kb < 2; kb++) {
+                                    mma_part<P>(s1[qq][kb], af, kf[kb][kc], sp_);
+                                    mma_part<P>(dp1[qq][kb], df, vf[kb][kc], sp_);
+                                }
+                        }
+                    S p[2][2][4], ds[2][2][4];
+#pragma unroll
+                    for (int qq = 0; qq < 2; qq++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
+                            const S mr = (S)sm[r], dr = (S)sd[r];
+                            const int lo = slo[r], hi = shi[r];
+#pragma unroll
+                            for (int kb = 0; kb < 2; kb++) {
+                                S x = s1[qq][kb][i] * scale2 - mr;
+                                if (!full) x = ((kj[kb] >= lo) & (kj[kb] < hi)) ? x : -INFINITY;
+                                const S pv = fast_exp2(x);
+                                p[qq][kb][i] = pv;
+                                ds[qq][kb][i] = pv * (dp1[qq][kb][i] - dr);
+                            }
+                        }
+                    Frag<P> pf1[2], dsf1[2];
+#pragma unroll
+                    for (int kb = 0; kb < 2; kb++) {
+                        S t1[8], t2[8];
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            t1[i] = p[0][kb][i]; t1[i + 4] = p[1][kb][i];
+                            t2[i] = ds[0][kb][i]; t2[i + 4] = ds[1][kb][i];
+                        }
+                        make_frag<P>(pf1[kb], t1);
+                        make_frag<P>(dsf1[kb], t2);
+                    }
+#pragma unroll
+                    for (int db = 0; db < DB; db++) {
+                        Frag<P> bo, bq;
+                        const int row = (db * 16 + c) * TP + hc * 32;
+                        tile_load_cperm<P>(bo, dotr, TPLANE, row, g);
+                        tile_load_cperm<P>(bq, qtr, TPLANE, row, g);
+#pragma unroll
+                        for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                            for (int kb = 0; kb < 2; kb++) {
+                                mma_part<P>(dv[kb][db], pf1[kb], bo, sp_);
+                                mma_part<P>(dk[kb][db], dsf1[kb], bq, sp_);
+                            }
+                    }
+                }
+        };
+        // One query tile's share of dK and dV, over its two 32-query halves
+        // h: S = Q K^T and dP = dO V^T (MFMA), then P and dS (VALU), then
+        // dV += P^T dO and dK += dS^T Q (MFMA).  FULL: every query of the
+        // tile sees every key of the block (no mask test).
+        //
+        // Below FP64 the stages run as a pipeline inside each wave, in
+        // steps that each pair a group of MFMAs with vector work that does
+        // not depend on them: S and dP of half 1 beside P and dS of half 0,
+        // dV and dK of half 0 beside P and dS of half 1.  An MFMA runs for
+        // many cycles after its issue, and the wave issues the step's VALU
+        // instructions meanwhile.  Each step's LDS operands are read one
+        // step ahead; a scheduling barrier closes each step, so the
+        // compiler keeps this order.
+        auto dkdv_tile = [&](auto full_c) __attribute__((always_inline)) {
+            constexpr bool FULL = decltype(full_c)::value;
+            AccV s[2][2][2], dp[2][2][2];   // [half][query block][key block]
+            Frag<P> pf[2][2], dsf[2][2];    // [half][key block]
+            // Row statistics of half hc, query block qq: rows crow(g, 0..3)
+            // are consecutive, so one 16-byte read per array (float rows).
+            St mrow[2][2][4], drow[2][2][4];
+            int lorow[2][2][4], hirow[2][2][4];
+            auto load_stats = [&](int hc) __attribute__((always_inline)) {
+#pragma unroll
+                for (int qq = 0; qq < 2; qq++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
+                        mrow[hc][qq][i] = sm[r];
+                        drow[hc][qq][i] = sd[r];
+                        if constexpr (!FULL) { lorow[hc][qq][i] = slo[r]; hirow[hc][qq][i] = shi[r]; }
+                    }
+            };
+            auto load_ad = [&](int hc, int kc, int qq, Frag<P> &af, Frag<P> &df) __attribute__((always_inline)) {
+                const int qb = 2 * hc + qq;
+                tile_load8<P>(af, qs, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
+                tile_load8<P>(df, dos, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
+            };
+            auto mma_sd = [&](int hc, int kc, int qq, const Frag<P> &af, const Frag<P> &df) __attribute__((always_inline)) {
+                if (kc == 0) {
+#pragma unroll
+                    for (int kb = 0; kb < 2; kb++) { s[hc][qq][kb] = AccV{0, 0, 0, 0}; dp[hc][qq][kb] = AccV{0, 0, 0, 0}; }
+                }
+#pragma unroll
+                for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                    for (int kb = 0; kb < 2; kb++) {
+                        mma_part<P>(s[hc][qq][kb], af, kf[kb][kc], sp_);
+                        mma_part<P>(dp[hc][qq][kb], df, vf[kb][kc], sp_);
+                    }
+            };
+            auto load_bo = [&](int hc, int db, Frag<P> &bo, Frag<P> &bq) __attribute__((always_inline)) {
+                const int row = (db * 16 + c) * TP + hc * 32;
+                tile_load_cperm<P>(bo, dotr, TPLANE, row, g);
+                tile_load_cperm<P>(bq, qtr, TPLANE, row, g);
+            };
+            auto mma_kv = [&](int hc, int db, const Frag<P> &bo, const Frag<P> &bq) __attribute__((always_inline)) {
+#pragma unroll
+                for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                    for (int kb = 0; kb < 2; kb++) {
+                        mma_part<P>(dv[kb][db], pf[hc][kb], bo, sp_);
+                        mma_part<P>(dk[kb][db], dsf[hc][kb], bq, sp_);
+                    }
+            };
+            // P = exp2(scale s log2(e) - m log2(e) - log2(l)) and dS =
+            // P (dP - delta) for rows crow(g, 2 ip), crow(g, 2 ip + 1) of
+            // query block qq, into fragment slots 4 qq + 2 ip, + 1.  A
+            // masked score gets exponent -inf: weight 0, no branch.
+            auto soft = [&](int hc, int qq, int ip) __attribute__((always_inline)) {
+#pragma unroll
+                for (int kb = 0; kb < 2; kb++) {
+                    S pv[2], dv_[2];
+#pragma unroll
+                    for (int t = 0; t < 2; t++) {
+                        const int i = 2 * ip + t;
+                        S x = s[hc][qq][kb][i] * scale2 - (S)mrow[hc][qq][i];
+                        if constexpr (!FULL)
+                            x = ((kj[kb] >= lorow[hc][qq][i]) & (kj[kb] < hirow[hc][qq][i])) ? x : -INFINITY;
+                        pv[t] = fast_exp2(x);
+                        dv_[t] = pv[t] * (dp[hc][qq][kb][i] - (S)drow[hc][qq][i]);
+                    }
+                    set_pair<P>(pf[hc][kb], 4 * qq + 2 * ip, pv[0], pv[1]);
+                    set_pair<P>(dsf[hc][kb], 4 * qq + 2 * ip, dv_[0], dv_[1]);
+                }
+            };
+            {
+                // Steps: S, dP of half 0 (phase A) and of half 1 (B), then
+                // dV, dK of half 0 (C) and of half 1 (D).  Step n + 1's LDS
+                // operands are read during step n.
+                constexpr int NA = 2 * KC, NC = DB, N = 2 * NA + 2 * NC;
+                auto load_step = [&](int n, Frag<P> &u, Frag<P> &v) __attribute__((always_inline)) {
+                    if (n < 2 * NA) load_ad(n / NA, (n % NA) / 2, n % 2, u, v);
+                    else load_bo((n - 2 * NA) / NC, (n - 2 * NA) % NC, u, v);
+                };
+                auto mma_step = [&](int n, const Frag<P> &u, const Frag<P> &v) __attribute__((always_inline)) {
+                    if (n < 2 * NA) mma_sd(n / NA, (n % NA) / 2, n % 2, u, v);
+                    else mma_kv((n - 2 * NA) / NC, (n - 2 * NA) % NC, u, v);
+                };
+                // P and dS of a half in four pieces (query block, row pair),
+                // spread evenly over the NS steps of the phase beside it.
+                auto pieces = [&](int hc, int m, int NS) __attribute__((always_inline)) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (k * NS / 4 == m) soft(hc, k / 2, k % 2);
+                };
+                Frag<P> u[2], v[2];
+                load_stats(0);
+                load_step(0, u[0], v[0]);
+#pragma unroll
+                for (int n = 0; n < N; n++) {
+                    if (n + 1 < N) load_step(n + 1, u[(n + 1) & 1], v[(n + 1) & 1]);
+                    mma_step(n, u[n & 1], v[n & 1]);
+                    if (n >= NA && n < 2 * NA) pieces(0, n - NA, NA);
+                    if (n >= 2 * NA && n < 2 * NA + NC) pieces(1, n - 2 * NA, NC);
+                    if (n == NA - 1) load_stats(1);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        };
         if (PF && qt_begin < qt_end) EF_FETCH(qt_begin);
         for (int64_t qt = qt_begin; qt < qt_end; qt++) {
             const int64_t q0 = qt * BN;
@@ -888,8 +1106,8 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                 any_tile = __any(sp.lo < sp.hi && sp.lo < kvalid && sp.hi > k0);
                 full_tile = __all(xq >= a.tq || (sp.lo <= k0 && sp.hi >= kvalid));
                 if (tid < BN) {
-                    slo[tid] = sp.lo;
-                    shi[tid] = sp.hi;
+                    slo[tid] = (int)sp.lo;
+                    shi[tid] = (int)sp.hi;
                     sm[tid] = xq < a.tq ? prm * (St)LOG2E + log2_of(prs) : St(0);
                     sd[tid] = xq < a.tq ? pdl : St(0);
                 }
@@ -897,94 +1115,14 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
             __syncthreads();
             if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
             if (any_tile) {
-            // The tile's queries in two 32-query halves (hc): S = Q K^T and
-            // dP = dO V^T for its two 16-query blocks, then P and dS, then
-            // dV += P^T dO and dK += dS^T Q over the half.
-#pragma unroll 1
-            for (int hc = 0; hc < 2; hc++) {
-                AccV s[2][2], dp[2][2];   // [query block][key block]
-#pragma unroll
-                for (int qq = 0; qq < 2; qq++)
-#pragma unroll
-                    for (int kb = 0; kb < 2; kb++) { s[qq][kb] = AccV{0, 0, 0, 0}; dp[qq][kb] = AccV{0, 0, 0, 0}; }
-#pragma unroll
-                for (int kc = 0; kc < KC; kc++) {
-#pragma unroll
-                    for (int qq = 0; qq < 2; qq++) {
-                        const int qb = 2 * hc + qq;
-                        Frag<P> af, df;
-                        tile_load8<P>(af, qs, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
-                        tile_load8<P>(df, dos, RPLANE, (qb * 16 + c) * RP + kc * 32 + g * 8);
-#pragma unroll
-                        for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
-#pragma unroll
-                            for (int kb = 0; kb < 2; kb++) {
-                                mma_part<P>(s[qq][kb], af, kf[kb][kc], sp_);
-                                mma_part<P>(dp[qq][kb], df, vf[kb][kc], sp_);
-                            }
-                    }
-                }
-                // P = exp2(scale s log2(e) - m log2(e) - log2(l)), dS =
-                // P (dP - delta), 0 where masked (no test on a full tile).
-                S p[2][2][4], ds[2][2][4];
-                if (full_tile) {
-#pragma unroll
-                    for (int qq = 0; qq < 2; qq++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
-                            const S mr = (S)sm[r], dr = (S)sd[r];
-#pragma unroll
-                            for (int kb = 0; kb < 2; kb++) {
-                                const S pv = fast_exp2(s[qq][kb][i] * scale2 - mr);
-                                p[qq][kb][i] = pv;
-                                ds[qq][kb][i] = pv * (dp[qq][kb][i] - dr);
-                            }
-                        }
-                } else {
-#pragma unroll
-                    for (int qq = 0; qq < 2; qq++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const int r = (2 * hc + qq) * 16 + crow<P>(g, i);
-                            const S mr = (S)sm[r], dr = (S)sd[r];
-                            const int64_t lo = slo[r], hi = shi[r];
-#pragma unroll
-                            for (int kb = 0; kb < 2; kb++) {
-                                const S e = fast_exp2(s[qq][kb][i] * scale2 - mr);
-                                const S pv = kj[kb] >= lo && kj[kb] < hi ? e : S(0);
-                                p[qq][kb][i] = pv;
-                                ds[qq][kb][i] = pv * (dp[qq][kb][i] - dr);
-                            }
-                        }
-                }
-                Frag<P> pf[2], dsf[2];
-#pragma unroll
-                for (int kb = 0; kb < 2; kb++) {
-                    S t1[8], t2[8];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        t1[i] = p[0][kb][i]; t1[i + 4] = p[1][kb][i];
-                        t2[i] = ds[0][kb][i]; t2[i + 4] = ds[1][kb][i];
-                    }
-                    make_frag<P>(pf[kb], t1);
-                    make_frag<P>(dsf[kb], t2);
-                }
-#pragma unroll
-                for (int db = 0; db < DB; db++) {
-                    Frag<P> bo, bq;
-                    const int row = (db * 16 + c) * TP + hc * 32;
-                    tile_load_cperm<P>(bo, dotr, TPLANE, row, g);
-                    tile_load_cperm<P>(bq, qtr, TPLANE, row, g);
-#pragma unroll
-                    for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
-#pragma unroll
-                        for (int kb = 0; kb < 2; kb++) {
-                            mma_part<P>(dv[kb][db], pf[kb], bo, sp_);
-                            mma_part<P>(dk[kb][db], dsf[kb], bq, sp_);
-                        }
-                }
-            }
+                // The tile's queries in two 32-query halves h, run as a
+                // pipeline inside each wave: S and dP of half 1 (MFMA)
+                // issued beside P and dS of half 0 (VALU), then dV and dK of
+                // half 0 beside P and dS of half 1, so that the matrix cores
+                // and the vector ALUs work at the same time.
+                if constexpr (P == PREC_F64) dkdv_tile_f64(full_tile);
+                else if (full_tile) dkdv_tile(std::true_type{});
+                else dkdv_tile(std::false_type{});
             }
             __syncthreads();
         }
@@ -1275,6 +1413,9 @@ int parts_for(const ef_attention_desc &d, int64_t blocks) {
 
 
 bool fast_bwd_shape(const ef_attention_desc &d) {
+    // The tile loads' 32-bit byte offsets: up to 32 rows of padded queries
+    // (8-byte elements) from a common base.
+    if ((size_t)round_up((int64_t)d.queries, PAD) >= (size_t(1) << 24)) return false;
     return d.features == d.values && (d.features == 32 || d.features == 64);
 }
 
@@ -1441,7 +1582,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         a.reverse = 0;   // key tiles in order: under a causal mask the first are the heaviest
         const int grid = a.items < cap ? a.items : cap;
         if (!(stages & 1)) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
-        hipLaunchKernelGGL((attn_bwd_dkdv_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
+        attn_bwd_launch(0, P, D, &a, (unsigned)grid, s);
         if (a.parts == 2) {
             const int64_t n = B * tk * D;
             hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(2 * n, 256)), dim3(256), 0, s,
@@ -1456,7 +1597,7 @@ ef_status run_bwd(const ef_attention_desc &d, const void *q, const void *k, cons
         a.reverse = d.causal ? 1 : 0;   // under a causal mask the last query tiles are the heaviest
         const int grid = a.items < cap ? a.items : cap;
         if (!(stages & 1)) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, a.ctr);
-        hipLaunchKernelGGL((attn_bwd_dq_kernel<P, D>), dim3((unsigned)grid), dim3(THREADS), 0, s, a);
+        attn_bwd_launch(1, P, D, &a, (unsigned)grid, s);
         if (a.parts == 2) {
             const int64_t n = B * tq * D;
             hipLaunchKernelGGL((attn_sum_kernel<S, In>), dim3(blocks_for(n, 256)), dim3(256), 0, s, (const S *)a.pdq, n,
@@ -1481,6 +1622,28 @@ ef_status dispatch_bwd(const ef_attention_desc &d, const void *q, const void *k,
 }
 
 }  // namespace
+
+#if EF_ATTN_BWD_TU
+template <int P, int D> void bwd_launch_one(int kernel, const void *args, unsigned grid, hipStream_t s) {
+    const BwdArgs a = *static_cast<const BwdArgs *>(args);
+    if (kernel == 0) hipLaunchKernelGGL((attn_bwd_dkdv_kernel<P, D>), dim3(grid), dim3(THREADS), 0, s, a);
+    else hipLaunchKernelGGL((attn_bwd_dq_kernel<P, D>), dim3(grid), dim3(THREADS), 0, s, a);
+}
+
+template <int P> void bwd_launch_p(int kernel, int d, const void *args, unsigned grid, hipStream_t s) {
+    if (d == 32) bwd_launch_one<P, 32>(kernel, args, grid, s);
+    else bwd_launch_one<P, 64>(kernel, args, grid, s);
+}
+
+void attn_bwd_launch(int kernel, int precision, int d, const void *args, unsigned grid, hipStream_t s) {
+    switch (precision) {
+    case PREC_F64: bwd_launch_p<PREC_F64>(kernel, d, args, grid, s); break;
+    case PREC_F32: bwd_launch_p<PREC_F32>(kernel, d, args, grid, s); break;
+    case PREC_BF16X3: bwd_launch_p<PREC_BF16X3>(kernel, d, args, grid, s); break;
+    default: bwd_launch_p<PREC_BF16>(kernel, d, args, grid, s); break;
+    }
+}
+#else
 
 size_t attention_scratch_bytes(const ef_attention_desc &d) { return layout(d).total; }
 
@@ -1550,5 +1713,7 @@ ef_status attention_bwd(const ef_attention_desc &d, const void *q, const void *k
         return EF_INVALID;
     }
 }
+
+#endif  // EF_ATTN_BWD_TU
 
 }  // namespace ef

@@ -110,32 +110,36 @@ __device__ __forceinline__ void mma_part(typename Acc<P>::type &c, const Frag<P>
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ double fast_exp2(double x) { return exp2(x); }
 
-// A fragment from 8 values in registers (split for BF16 precisions).
-template <int P, class T> __device__ __forceinline__ void make_frag(Frag<P> &f, const T (&x)[8]) {
+// Slots j and j + 1 (j even) of a fragment from two values (split for
+// BF16 precisions: two values per conversion, hi read back from the packed
+// bits).
+template <int P, class T> __device__ __forceinline__ void set_pair(Frag<P> &f, int j, T y0, T y1) {
     if constexpr (P == PREC_F32) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) f.v[j] = (float)x[j];
+        f.v[j] = (float)y0;
+        f.v[j + 1] = (float)y1;
     } else if constexpr (P == PREC_F64) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) f.v[j] = (double)x[j];
+        f.v[j] = (double)y0;
+        f.v[j + 1] = (double)y1;
     } else if constexpr (P == PREC_BF16) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) f.hi[j] = (__bf16)(float)x[j];
+        f.hi[j] = (__bf16)(float)y0;
+        f.hi[j + 1] = (__bf16)(float)y1;
     } else {
-        // Two values per conversion; hi read back from the packed bits.
-#pragma unroll
-        for (int j = 0; j < 8; j += 2) {
-            const float x0 = (float)x[j], x1 = (float)x[j + 1];
-            const bf16x2 h = __builtin_convertvector((f32x2){x0, x1}, bf16x2);
-            const unsigned u = __builtin_bit_cast(unsigned, h);
-            const f32x2 r = {x0 - __uint_as_float(u << 16), x1 - __uint_as_float(u & 0xffff0000u)};
-            const bf16x2 l = __builtin_convertvector(r, bf16x2);
-            f.hi[j] = h[0];
-            f.hi[j + 1] = h[1];
-            f.lo[j] = l[0];
-            f.lo[j + 1] = l[1];
-        }
+        const float x0 = (float)y0, x1 = (float)y1;
+        const bf16x2 h = __builtin_convertvector((f32x2){x0, x1}, bf16x2);
+        const unsigned u = __builtin_bit_cast(unsigned, h);
+        const f32x2 r = {x0 - __uint_as_float(u << 16), x1 - __uint_as_float(u & 0xffff0000u)};
+        const bf16x2 l = __builtin_convertvector(r, bf16x2);
+        f.hi[j] = h[0];
+        f.hi[j + 1] = h[1];
+        f.lo[j] = l[0];
+        f.lo[j + 1] = l[1];
     }
+}
+
+// A fragment from 8 values in registers.
+template <int P, class T> __device__ __forceinline__ void make_frag(Frag<P> &f, const T (&x)[8]) {
+#pragma unroll
+    for (int j = 0; j < 8; j += 2) set_pair<P>(f, j, x[j], x[j + 1]);
 }
 
 // Stores one value into a tile of `plane` elements per plane.
