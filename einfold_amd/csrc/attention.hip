@@ -1088,43 +1088,63 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                 }
             }
         };
-        if (PF && qt_begin < qt_end) EF_FETCH(qt_begin);
-        for (int64_t qt = qt_begin; qt < qt_end; qt++) {
-            const int64_t q0 = qt * BN;
-            if (!PF) EF_FETCH(qt);
-            RowT::store(qs, RP, RPLANE, pq);
-            ColT::store(qtr, TP, TPLANE, pqt);
-            RowT::store(dos, RP, RPLANE, pd);
-            ColT::store(dotr, TP, TPLANE, pdt);
-            // Every wave votes over the tile's 64 queries (one per lane), so
-            // all agree without a barrier: does any see a key of the block,
-            // does each see all of them.
-            bool any_tile, full_tile;
-            {
-                const int64_t xq = q0 + lane;
-                const Span sp = query_span(a.mask, xq);
-                any_tile = __any(sp.lo < sp.hi && sp.lo < kvalid && sp.hi > k0);
-                full_tile = __all(xq >= a.tq || (sp.lo <= k0 && sp.hi >= kvalid));
-                if (tid < BN) {
-                    slo[tid] = (int)sp.lo;
-                    shi[tid] = (int)sp.hi;
-                    sm[tid] = xq < a.tq ? prm * (St)LOG2E + log2_of(prs) : St(0);
-                    sd[tid] = xq < a.tq ? pdl : St(0);
+        // Tiles [qa, qb): FULL (a compile-time choice) runs the pipeline
+        // without mask tests and needs every query of each tile to see every
+        // key of the block; otherwise each tile's flags choose.
+        auto run_tiles = [&](auto full_c, int64_t qa, int64_t qb) __attribute__((always_inline)) {
+            constexpr bool FULL = decltype(full_c)::value;
+            for (int64_t qt = qa; qt < qb; qt++) {
+                const int64_t q0 = qt * BN;
+                if (!PF) EF_FETCH(qt);
+                RowT::store(qs, RP, RPLANE, pq);
+                ColT::store(qtr, TP, TPLANE, pqt);
+                RowT::store(dos, RP, RPLANE, pd);
+                ColT::store(dotr, TP, TPLANE, pdt);
+                // Every wave votes over the tile's 64 queries (one per lane),
+                // so all agree without a barrier: does any see a key of the
+                // block, does each see all of them.
+                bool any_tile, full_tile;
+                {
+                    const int64_t xq = q0 + lane;
+                    const Span sp = query_span(a.mask, xq);
+                    any_tile = FULL || __any(sp.lo < sp.hi && sp.lo < kvalid && sp.hi > k0);
+                    full_tile = FULL || __all(xq >= a.tq || (sp.lo <= k0 && sp.hi >= kvalid));
+                    if (tid < BN) {
+                        slo[tid] = (int)sp.lo;
+                        shi[tid] = (int)sp.hi;
+                        sm[tid] = xq < a.tq ? prm * (St)LOG2E + log2_of(prs) : St(0);
+                        sd[tid] = xq < a.tq ? pdl : St(0);
+                    }
                 }
+                __syncthreads();
+                if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
+                if (any_tile) {
+                    if constexpr (P == PREC_F64) dkdv_tile_f64(full_tile);
+                    else if constexpr (FULL) dkdv_tile(std::true_type{});
+                    else if (full_tile) dkdv_tile(std::true_type{});
+                    else dkdv_tile(std::false_type{});
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
-            if (any_tile) {
-                // The tile's queries in two 32-query halves h, run as a
-                // pipeline inside each wave: S and dP of half 1 (MFMA)
-                // issued beside P and dS of half 0 (VALU), then dV and dK of
-                // half 0 beside P and dS of half 1, so that the matrix cores
-                // and the vector ALUs work at the same time.
-                if constexpr (P == PREC_F64) dkdv_tile_f64(full_tile);
-                else if (full_tile) dkdv_tile(std::true_type{});
-                else dkdv_tile(std::false_type{});
-            }
-            __syncthreads();
+        };
+        // Where every tile from some point on is full (a causal mask alone:
+        // from the first tile whose first query sees the block's last key;
+        // no mask: all), those tiles run in a loop of their own, so the
+        // loop's code holds one variant of the pipeline.
+        int64_t qt_full = qt_end;
+        if (!a.mask.allowed && !a.mask.causal) {
+            qt_full = qt_begin;
+        } else if (a.causal_only) {
+            const int64_t need = kvalid - 1 - a.mask.offset;   // first query that sees key kvalid - 1
+            const int64_t t = need <= 0 ? 0 : (need + BN - 1) / BN;
+            qt_full = t < qt_begin ? qt_begin : (t > qt_end ? qt_end : t);
+        }
+        if (PF && qt_begin < qt_end) EF_FETCH(qt_begin);
+        if constexpr (P == PREC_F64) {
+            run_tiles(std::false_type{}, qt_begin, qt_end);
+        } else {
+            run_tiles(std::false_type{}, qt_begin, qt_full);
+            run_tiles(std::true_type{}, qt_full, qt_end);
         }
 #undef EF_FETCH
         // Lane holds dK/dV[key k0 + w*32 + kb*16 + crow(g, i)][d = db*16 + c].
