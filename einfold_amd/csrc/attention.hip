@@ -1225,12 +1225,15 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
 
         int64_t x[2];
         Span sp[2];
+        int slo32[2], shi32[2];   // spans in 32 bits (tk < 2^31)
         Frag<P> qf[2][KC], df[2][KC];
         S mrow[2], drow[2];
 #pragma unroll
         for (int qb = 0; qb < 2; qb++) {
             x[qb] = q0 + w * 32 + qb * 16 + c;
             sp[qb] = query_span(a.mask, x[qb]);
+            slo32[qb] = (int)sp[qb].lo;
+            shi32[qb] = (int)sp[qb].hi;
 #pragma unroll
             for (int kc = 0; kc < KC; kc++) {
                 In t1[8], t2[8];
@@ -1284,22 +1287,9 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
         ColT::load(pkt, KT + (J_), a.tkp, tplane);                                           \
         RowT::load(pv_, VR + (J_) * D, D, rplane);                                           \
     } while (0)
-        if (PF && jbeg < last) EF_FETCH(jbeg);
-        for (int64_t j0 = jbeg; j0 < last; j0 += BN) {
-            if (!PF) EF_FETCH(j0);
-            RowT::store(ks, RP, RPLANE, pk);
-            ColT::store(ktr, TP, TPLANE, pkt);
-            RowT::store(vs, RP, RPLANE, pv_);
-            // Whether every query of the workgroup sees every real key of
-            // the tile (then no mask test per element).
-            const int64_t jvalid = j0 + BN < a.tk ? j0 + BN : a.tk;
-            bool full = true;
-#pragma unroll
-            for (int qb = 0; qb < 2; qb++) full &= x[qb] >= a.tq || (sp[qb].lo <= j0 && sp[qb].hi >= jvalid);
-            // Each wave picks the mask-free path for its own queries.
-            const bool full_tile = __all(full);
-            __syncthreads();
-            if (PF && j0 + BN < last) EF_FETCH(j0 + BN);
+        // FP64: each half's stages one after another, the mask tested
+        // unless the tile is full.
+        auto dq_tile_f64 = [&](bool full_tile, int64_t j0) __attribute__((always_inline)) {
             // The tile's keys in two 32-key halves: S^T = K Q^T and
             // dP^T = V dO^T, then dS^T, then dQ^T += K^T dS^T.
 #pragma unroll 1
@@ -1361,7 +1351,136 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
 #pragma unroll
                         for (int qb = 0; qb < 2; qb++) mma_part<P>(dq[qb][db], kt[db], dsf[qb], sp_);
             }
-            __syncthreads();
+        };
+        // One key tile's share of dQ, over its two 32-key halves h: S^T =
+        // K Q^T and dP^T = V dO^T (MFMA), then dS^T (VALU), then dQ^T +=
+        // K^T dS^T (MFMA), as a pipeline of steps inside each wave (the
+        // dK/dV kernel's schedule): S^T and dP^T of half 1 beside dS^T of
+        // half 0, dQ^T of half 0 beside dS^T of half 1.  FULL: every query
+        // of the wave sees every key of the tile.
+        auto dq_tile = [&](auto full_c, int64_t j0) __attribute__((always_inline)) {
+            constexpr bool FULL = decltype(full_c)::value;
+            AccV s[2][2][2], dp[2][2][2];   // [half][key block][query block]
+            Frag<P> dsf[2][2];              // [half][query block]
+            auto load_kv = [&](int hc, int kc, int kk, Frag<P> &af, Frag<P> &vf) __attribute__((always_inline)) {
+                const int kb = 2 * hc + kk;
+                tile_load8<P>(af, ks, RPLANE, (kb * 16 + c) * RP + kc * 32 + g * 8);
+                tile_load8<P>(vf, vs, RPLANE, (kb * 16 + c) * RP + kc * 32 + g * 8);
+            };
+            auto mma_sd = [&](int hc, int kc, int kk, const Frag<P> &af, const Frag<P> &vf) __attribute__((always_inline)) {
+                if (kc == 0) {
+#pragma unroll
+                    for (int qb = 0; qb < 2; qb++) { s[hc][kk][qb] = AccV{0, 0, 0, 0}; dp[hc][kk][qb] = AccV{0, 0, 0, 0}; }
+                }
+#pragma unroll
+                for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                    for (int qb = 0; qb < 2; qb++) {
+                        mma_part<P>(s[hc][kk][qb], af, qf[qb][kc], sp_);
+                        mma_part<P>(dp[hc][kk][qb], vf, df[qb][kc], sp_);
+                    }
+            };
+            auto load_kt = [&](int hc, int db, Frag<P> &kt) __attribute__((always_inline)) {
+                tile_load_cperm<P>(kt, ktr, TPLANE, (db * 16 + c) * TP + hc * 32, g);
+            };
+            auto mma_dq = [&](int hc, int db, const Frag<P> &kt) __attribute__((always_inline)) {
+#pragma unroll
+                for (int sp_ = 0; sp_ < Parts<P>::n; sp_++)
+#pragma unroll
+                    for (int qb = 0; qb < 2; qb++) mma_part<P>(dq[qb][db], kt, dsf[hc][qb], sp_);
+            };
+            // dS^T of key block kk, query block qb of half hc: fragment
+            // slots 4 kk .. 4 kk + 3.  A masked score gets exponent -inf.
+            auto soft = [&](int hc, int qb, int kk) __attribute__((always_inline)) {
+                S t[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    S xs = s[hc][kk][qb][i] * scale2 - mrow[qb];
+                    if constexpr (!FULL) {
+                        const int j = (int)j0 + (2 * hc + kk) * 16 + crow<P>(g, i);
+                        xs = ((j >= slo32[qb]) & (j < shi32[qb])) ? xs : -INFINITY;
+                    }
+                    t[i] = fast_exp2(xs) * (dp[hc][kk][qb][i] - drow[qb]);
+                }
+                set_pair<P>(dsf[hc][qb], 4 * kk, t[0], t[1]);
+                set_pair<P>(dsf[hc][qb], 4 * kk + 2, t[2], t[3]);
+            };
+            constexpr int NA = 2 * KC, NC = DB, N = 2 * NA + 2 * NC;
+            auto load_step = [&](int n, Frag<P> &u, Frag<P> &v) __attribute__((always_inline)) {
+                if (n < 2 * NA) load_kv(n / NA, (n % NA) / 2, n % 2, u, v);
+                else load_kt((n - 2 * NA) / NC, (n - 2 * NA) % NC, u);
+            };
+            auto mma_step = [&](int n, const Frag<P> &u, const Frag<P> &v) __attribute__((always_inline)) {
+                if (n < 2 * NA) mma_sd(n / NA, (n % NA) / 2, n % 2, u, v);
+                else mma_dq((n - 2 * NA) / NC, (n - 2 * NA) % NC, u);
+            };
+            auto pieces = [&](int hc, int m, int NS) __attribute__((always_inline)) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k * NS / 4 == m) soft(hc, k / 2, k % 2);
+            };
+            Frag<P> u[2], v[2];
+            load_step(0, u[0], v[0]);
+#pragma unroll
+            for (int n = 0; n < N; n++) {
+                if (n + 1 < N) load_step(n + 1, u[(n + 1) & 1], v[(n + 1) & 1]);
+                mma_step(n, u[n & 1], v[n & 1]);
+                if (n >= NA && n < 2 * NA) pieces(0, n - NA, NA);
+                if (n >= 2 * NA && n < 2 * NA + NC) pieces(1, n - 2 * NA, NC);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        };
+        // Key tiles [ja, jb): FULL (a compile-time choice) needs every query
+        // of the workgroup to see every key of each tile.
+        auto run_tiles = [&](auto full_c, int64_t ja, int64_t jb) __attribute__((always_inline)) {
+            constexpr bool FULL = decltype(full_c)::value;
+            for (int64_t j0 = ja; j0 < jb; j0 += BN) {
+                if (!PF) EF_FETCH(j0);
+                RowT::store(ks, RP, RPLANE, pk);
+                ColT::store(ktr, TP, TPLANE, pkt);
+                RowT::store(vs, RP, RPLANE, pv_);
+                // Whether every query of the wave sees every real key of the
+                // tile (then no mask test per element).
+                bool full_tile = true;
+                if constexpr (!FULL) {
+                    const int64_t jvalid = j0 + BN < a.tk ? j0 + BN : a.tk;
+                    bool full = true;
+#pragma unroll
+                    for (int qb = 0; qb < 2; qb++) full &= x[qb] >= a.tq || (sp[qb].lo <= j0 && sp[qb].hi >= jvalid);
+                    full_tile = __all(full);
+                }
+                __syncthreads();
+                if (PF && j0 + BN < last) EF_FETCH(j0 + BN);
+                if constexpr (P == PREC_F64) dq_tile_f64(full_tile, j0);
+                else if constexpr (FULL) dq_tile(std::true_type{}, j0);
+                else if (full_tile) dq_tile(std::true_type{}, j0);
+                else dq_tile(std::false_type{}, j0);
+                __syncthreads();
+            }
+        };
+        // Under a causal mask alone the first key tiles are seen whole by
+        // every query of the block (a tile is, when its last key is at most
+        // q0 + offset); with no mask, all are.  Those run in a loop of their
+        // own that holds one variant of the pipeline.
+        int64_t j_full = jbeg;
+        if (!a.mask.allowed && !a.mask.causal) {
+            j_full = last;
+        } else if (a.causal_only) {
+            const int64_t lim = q0 + a.mask.offset + 1;   // query q0 sees keys below lim
+            if (lim >= a.tk) {
+                j_full = last;
+            } else {
+                const int64_t n = lim - BN - jbeg;
+                j_full = n < 0 ? jbeg : jbeg + (n / BN + 1) * BN;
+            }
+            j_full = j_full < jbeg ? jbeg : (j_full > last ? last : j_full);
+        }
+        if (PF && jbeg < last) EF_FETCH(jbeg);
+        if constexpr (P == PREC_F64) {
+            run_tiles(std::false_type{}, jbeg, last);
+        } else {
+            run_tiles(std::true_type{}, jbeg, j_full);
+            run_tiles(std::false_type{}, j_full, last);
         }
 #undef EF_FETCH
         // Lane holds dQ^T[d = db*16 + crow(g, i)][query x].
