@@ -270,21 +270,17 @@ template <class E, int PLN, int R, int C> struct TileIO {
     // and row block of chunk i) and this thread's 32-bit offset, so that
     // each load is a scalar base plus one register.
     static constexpr bool SPLIT = PER_PLANE % THREADS == 0 && THREADS % PER_ROW == 0;
-    __device__ static __forceinline__ void load(u32x4 (&buf)[N], const E *g, int64_t gstride, int64_t gplane) {
+    // Chunk i alone (a caller may spread a tile's loads over its work).
+    __device__ static __forceinline__ void load_chunk(u32x4 (&buf)[N], int i, const E *g, int64_t gstride,
+                                                      int64_t gplane) {
         if constexpr (SPLIT) {
             const unsigned lane_off = (unsigned)(((int)threadIdx.x / PER_ROW) * gstride +
                                                  ((int)threadIdx.x % PER_ROW) * CE) * (unsigned)sizeof(E);
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                constexpr int BLOCKS = PER_PLANE / THREADS;
-                const int p = i / BLOCKS, rows = (i % BLOCKS) * (THREADS / PER_ROW);
-                const char *base = reinterpret_cast<const char *>(g + p * gplane + rows * gstride);
-                buf[i] = *reinterpret_cast<const u32x4 *>(base + (uint64_t)lane_off);
-            }
-            return;
-        }
-#pragma unroll
-        for (int i = 0; i < N; i++) {
+            constexpr int BLOCKS = PER_PLANE / THREADS;
+            const int p = i / BLOCKS, rows = (i % BLOCKS) * (THREADS / PER_ROW);
+            const char *base = reinterpret_cast<const char *>(g + p * gplane + rows * gstride);
+            buf[i] = *reinterpret_cast<const u32x4 *>(base + (uint64_t)lane_off);
+        } else {
             const int k = (int)threadIdx.x + THREADS * i;
             if (TOTAL % THREADS == 0 || k < TOTAL) {
                 const int p = k / PER_PLANE, rem = k % PER_PLANE;
@@ -292,6 +288,10 @@ template <class E, int PLN, int R, int C> struct TileIO {
                 buf[i] = *reinterpret_cast<const u32x4 *>(g + p * gplane + r * gstride + cc);
             }
         }
+    }
+    __device__ static __forceinline__ void load(u32x4 (&buf)[N], const E *g, int64_t gstride, int64_t gplane) {
+#pragma unroll
+        for (int i = 0; i < N; i++) load_chunk(buf, i, g, gstride, gplane);
     }
     __device__ static __forceinline__ void store(E *lds, int lpitch, int lplane, const u32x4 (&buf)[N]) {
 #pragma unroll
@@ -898,6 +898,30 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         RowT::load(pd, DR + q0_ * D, D, rplane);                                             \
         ColT::load(pdt, DT + q0_, a.tqp, tplane);                                            \
     } while (0)
+        // Part k of the same loads for tile T, so that the pipelined tile
+        // spreads them over its steps: 0 the row statistics, then the chunks
+        // of Q rows, Q^T, dO rows and dO^T in turn.
+        constexpr int FETCH_PARTS = 1 + 2 * (RowT::N + ColT::N);
+        auto fetch_part = [&](int64_t T, int k) __attribute__((always_inline)) {
+            const int64_t q0_ = T * BN;
+            if (k == 0) {
+                if (tid < BN) {
+                    const int64_t xc_ = q0_ + tid < a.tq ? q0_ + tid : a.tq - 1;
+                    prm = RM[xc_];
+                    prs = RS[xc_];
+                    pdl = DL[xc_];
+                }
+                return;
+            }
+            int i = k - 1;
+            if (i < RowT::N) { RowT::load_chunk(pq, i, QR + q0_ * D, D, rplane); return; }
+            i -= RowT::N;
+            if (i < ColT::N) { ColT::load_chunk(pqt, i, QT + q0_, a.tqp, tplane); return; }
+            i -= ColT::N;
+            if (i < RowT::N) { RowT::load_chunk(pd, i, DR + q0_ * D, D, rplane); return; }
+            i -= RowT::N;
+            ColT::load_chunk(pdt, i, DT + q0_, a.tqp, tplane);
+        };
         // FP64: each half's stages one after another, the mask tested
         // unless the tile is full.
         auto dkdv_tile_f64 = [&](bool full) __attribute__((always_inline)) {
@@ -982,7 +1006,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
         // instructions meanwhile.  Each step's LDS operands are read one
         // step ahead; a scheduling barrier closes each step, so the
         // compiler keeps this order.
-        auto dkdv_tile = [&](auto full_c) __attribute__((always_inline)) {
+        auto dkdv_tile = [&](auto full_c, int64_t next) __attribute__((always_inline)) {
             constexpr bool FULL = decltype(full_c)::value;
             AccV s[2][2][2], dp[2][2][2];   // [half][query block][key block]
             Frag<P> pf[2][2], dsf[2][2];    // [half][key block]
@@ -1084,6 +1108,15 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                     if (n >= NA && n < 2 * NA) pieces(0, n - NA, NA);
                     if (n >= 2 * NA && n < 2 * NA + NC) pieces(1, n - 2 * NA, NC);
                     if (n == NA - 1) load_stats(1);
+                    // The next tile's loads (with PF), a few per step of
+                    // the last phase (pure MFMA, the fewest live values):
+                    // the memory pipeline takes them while the wave computes
+                    // instead of holding the wave at one burst of 17.
+                    if constexpr (PF) {
+#pragma unroll
+                        for (int k = 0; k < FETCH_PARTS; k++)
+                            if (2 * NA + NC + k * NC / FETCH_PARTS == n) fetch_part(next, k);
+                    }
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
@@ -1117,12 +1150,22 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dkdv_kernel(BwdArgs a) {
                     }
                 }
                 __syncthreads();
-                if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
-                if (any_tile) {
-                    if constexpr (P == PREC_F64) dkdv_tile_f64(full_tile);
-                    else if constexpr (FULL) dkdv_tile(std::true_type{});
-                    else if (full_tile) dkdv_tile(std::true_type{});
-                    else dkdv_tile(std::false_type{});
+                // The pipelined tile issues the next tile's loads itself (the
+                // last tile reloads its own: harmless, and no branch).
+                const int64_t nxt = qt + 1 < qt_end ? qt + 1 : qt;
+                if constexpr (P == PREC_F64) {
+                    if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
+                    if (any_tile) dkdv_tile_f64(full_tile);
+                } else if constexpr (FULL) {
+                    dkdv_tile(std::true_type{}, nxt);
+                } else {
+                    if (!any_tile) {
+                        if (PF && qt + 1 < qt_end) EF_FETCH(qt + 1);
+                    } else if (full_tile) {
+                        dkdv_tile(std::true_type{}, nxt);
+                    } else {
+                        dkdv_tile(std::false_type{}, nxt);
+                    }
                 }
                 __syncthreads();
             }
