@@ -1330,6 +1330,16 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
         ColT::load(pkt, KT + (J_), a.tkp, tplane);                                           \
         RowT::load(pv_, VR + (J_) * D, D, rplane);                                           \
     } while (0)
+        // Chunk k of the same loads for the tile at J, spread by the
+        // pipelined tile over its last phase: K rows, K^T, V rows in turn.
+        constexpr int FETCH_PARTS = 2 * RowT::N + ColT::N;
+        auto fetch_part = [&](int64_t J, int k) __attribute__((always_inline)) {
+            if (k < RowT::N) { RowT::load_chunk(pk, k, KR + J * D, D, rplane); return; }
+            k -= RowT::N;
+            if (k < ColT::N) { ColT::load_chunk(pkt, k, KT + J, a.tkp, tplane); return; }
+            k -= ColT::N;
+            RowT::load_chunk(pv_, k, VR + J * D, D, rplane);
+        };
         // FP64: each half's stages one after another, the mask tested
         // unless the tile is full.
         auto dq_tile_f64 = [&](bool full_tile, int64_t j0) __attribute__((always_inline)) {
@@ -1401,7 +1411,7 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
         // dK/dV kernel's schedule): S^T and dP^T of half 1 beside dS^T of
         // half 0, dQ^T of half 0 beside dS^T of half 1.  FULL: every query
         // of the wave sees every key of the tile.
-        auto dq_tile = [&](auto full_c, int64_t j0) __attribute__((always_inline)) {
+        auto dq_tile = [&](auto full_c, int64_t j0, int64_t next) __attribute__((always_inline)) {
             constexpr bool FULL = decltype(full_c)::value;
             AccV s[2][2][2], dp[2][2][2];   // [half][key block][query block]
             Frag<P> dsf[2][2];              // [half][query block]
@@ -1470,6 +1480,13 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
                 mma_step(n, u[n & 1], v[n & 1]);
                 if (n >= NA && n < 2 * NA) pieces(0, n - NA, NA);
                 if (n >= 2 * NA && n < 2 * NA + NC) pieces(1, n - 2 * NA, NC);
+                // The next tile's loads (with PF), a few per step of the last
+                // phase, instead of one burst after the barrier.
+                if constexpr (PF) {
+#pragma unroll
+                    for (int k = 0; k < FETCH_PARTS; k++)
+                        if (2 * NA + NC + k * NC / FETCH_PARTS == n) fetch_part(next, k);
+                }
                 __builtin_amdgcn_sched_barrier(0);
             }
         };
@@ -1493,11 +1510,19 @@ __global__ __launch_bounds__(THREADS, 1) void attn_bwd_dq_kernel(BwdArgs a) {
                     full_tile = __all(full);
                 }
                 __syncthreads();
-                if (PF && j0 + BN < last) EF_FETCH(j0 + BN);
-                if constexpr (P == PREC_F64) dq_tile_f64(full_tile, j0);
-                else if constexpr (FULL) dq_tile(std::true_type{}, j0);
-                else if (full_tile) dq_tile(std::true_type{}, j0);
-                else dq_tile(std::false_type{}, j0);
+                // The pipelined tile issues the next tile's loads itself (the
+                // last tile reloads its own: harmless, and no branch).
+                const int64_t nxt = j0 + BN < last ? j0 + BN : j0;
+                if constexpr (P == PREC_F64) {
+                    if (PF && j0 + BN < last) EF_FETCH(j0 + BN);
+                    dq_tile_f64(full_tile, j0);
+                } else if constexpr (FULL) {
+                    dq_tile(std::true_type{}, j0, nxt);
+                } else if (full_tile) {
+                    dq_tile(std::true_type{}, j0, nxt);
+                } else {
+                    dq_tile(std::false_type{}, j0, nxt);
+                }
                 __syncthreads();
             }
         };
