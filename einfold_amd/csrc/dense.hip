@@ -101,16 +101,18 @@ __global__ __launch_bounds__(256) void dense_accumulate_kernel(const double *__r
 
 // Outputs too small to fill the chip with the tiles above (256^3 makes 64
 // workgroups of 32 x 32): one cell per thread, 16 x 16 cells per
-// workgroup, z in chunks of 64 staged through LDS, the next chunk loaded
-// into registers while this one is summed.  Each cell's chain is the same
-// z-ordered sequence of roundings, so the same bits.
+// workgroup, z in chunks of 256 staged through LDS (69 KB), the next chunk
+// loaded into registers while this one is summed.  A chunk's chain of 256
+// sums lasts about one memory latency, so the next chunk's loads are hidden
+// behind it, and 256^3 takes one exposed load in all.  Each cell's chain is
+// the same z-ordered sequence of roundings, so the same bits.
 template <bool PRED>
 __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__restrict__ a,
                                                              const uint32_t *__restrict__ held,
                                                              const double *__restrict__ b, int64_t nx, int64_t nz,
                                                              int64_t ny, int64_t per_batch, double *__restrict__ out) {
 #pragma clang fp contract(off)
-    constexpr int T = 16, ZT = 64, PER = T * ZT / 256;   // 4 values of a and of b per thread per chunk
+    constexpr int T = 16, ZT = 256, PER = T * ZT / 256;   // 16 values of a and of b per thread per chunk
     __shared__ double as[T][ZT + 1];
     __shared__ uint8_t fs[T][ZT + 1];
     __shared__ double bs[ZT][T];
@@ -155,7 +157,26 @@ __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__res
         __syncthreads();
         if (z0 + ZT < nz) fetch(z0 + ZT);
         const int zn = nz - z0 < ZT ? (int)(nz - z0) : ZT;
-        for (int zz = 0; zz < zn; zz++) {
+        // 16 z at a time: every LDS read of the run issued before the chain
+        // of sums that uses them, so the chain waits on one read latency per
+        // run instead of one per z.
+        int zz0 = 0;
+        for (; zz0 + 16 <= zn; zz0 += 16) {
+            double av[16], bv[16];
+            bool fv[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                av[j] = as[ty][zz0 + j];
+                bv[j] = bs[zz0 + j][tx];
+                fv[j] = !PRED || fs[ty][zz0 + j];
+            }
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const double t = acc + av[j] * bv[j];
+                acc = fv[j] ? t : acc;
+            }
+        }
+        for (int zz = zz0; zz < zn; zz++) {
             const double t = acc + as[ty][zz] * bs[zz][tx];
             acc = (!PRED || fs[ty][zz]) ? t : acc;
         }
