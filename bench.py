@@ -394,6 +394,14 @@ def large_fold_line(dev):
                            "frac_of_issued_peak": round(issued * tf / (peak / 1e12), 4),
                            "rel_err_8_rows": err}
         plan.close()
+    # The vendor library's GEMM (torch.matmul, hipBLASLt) on the same
+    # operands: what this GPU sustains on a plain product of this size, for
+    # scale next to the published peaks.
+    for name, dt in (("library_bf16", torch.bfloat16), ("library_fp32", torch.float32)):
+        x, y = a.to(dt), b.to(dt)
+        lib_ms = _events(lambda: torch.matmul(x, y), 5)
+        line[name] = {"ms": round(lib_ms, 3), "tflops": round(flop / (lib_ms * 1e-3) / 1e12, 1)}
+        del x, y
     return line
 
 

@@ -170,16 +170,13 @@ __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__res
                 bv[j] = bs[zz0 + j][tx];
                 fv[j] = !PRED || fs[ty][zz0 + j];
             }
+            // A skipped cell adds -0.0, which leaves every sum as it is
+            // (x + -0.0 == x, signed zeros included): the select is taken on
+            // the product, off the chain of dependent sums.
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const double t = acc + av[j] * bv[j];
-                acc = fv[j] ? t : acc;
-            }
+            for (int j = 0; j < 16; j++) acc = acc + (fv[j] ? av[j] * bv[j] : -0.0);
         }
-        for (int zz = zz0; zz < zn; zz++) {
-            const double t = acc + as[ty][zz] * bs[zz][tx];
-            acc = (!PRED || fs[ty][zz]) ? t : acc;
-        }
+        for (int zz = zz0; zz < zn; zz++) acc = acc + ((!PRED || fs[ty][zz]) ? as[ty][zz] * bs[zz][tx] : -0.0);
         __syncthreads();
     }
     const int64_t x = x0 + ty, y = y0 + tx;
