@@ -6,6 +6,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <unistd.h>
 
 #include <hip/hip_runtime.h>
 #include <new>
@@ -121,13 +122,24 @@ bool read_relation(const ef_relation *in, Relation &out) {
 // staged in chunks (and, fresh, take a page fault on first touch of each
 // page: about 13x slower for hundreds of MB).  Pinning a buffer
 // (hipHostMalloc) costs far more than copying into it, so released pinned
-// buffers are kept in a pool by size class and handed out again, up
-// to PIN_POOL_CAP bytes; the pool lives until the process ends (its
-// buffers are never freed after the runtime has shut down).  Smaller
-// buffers stay pageable.
+// buffers are kept in a pool by size class and handed out again, up to
+// pin_pool_cap() bytes (a sixteenth of the host's memory, at most 16 GiB:
+// room for the outputs of one call of a few hundred million rows); the
+// pool lives until the process ends (its buffers are never freed after the
+// runtime has shut down).  Smaller buffers stay pageable.
 namespace {
 constexpr size_t PIN_FROM = (size_t)64 << 10;
-constexpr size_t PIN_POOL_CAP = (size_t)1 << 30;
+
+size_t pin_pool_cap() {
+    static const size_t cap = [] {
+        const long pages = sysconf(_SC_PHYS_PAGES), page = sysconf(_SC_PAGESIZE);
+        const size_t host = pages > 0 && page > 0 ? (size_t)pages * (size_t)page : 0;
+        const size_t sixteenth = host / 16;
+        const size_t most = (size_t)16 << 30, least = (size_t)1 << 30;
+        return sixteenth > most ? most : (sixteenth < least ? least : sixteenth);
+    }();
+    return cap;
+}
 
 struct PinnedPool {
     std::mutex mu;
@@ -188,7 +200,7 @@ HostBuffer::~HostBuffer() {
     PinnedPool &pool = pinned_pool();
     {
         std::lock_guard<std::mutex> lock(pool.mu);
-        if (pool.cached + capacity <= PIN_POOL_CAP) {
+        if (pool.cached + capacity <= pin_pool_cap()) {
             pool.free.emplace(capacity, data);
             pool.cached += capacity;
             return;

@@ -204,11 +204,25 @@ bool fold_schema(const FoldModel &m, const std::vector<const Relation *> &operan
     return true;
 }
 
+// Device chunks kept between calls, up to a quarter of the device's memory
+// in all: allocating device memory that was freed shortly before can cost
+// a second or more per GiB (the driver clears it first), so a workload
+// that repeats large calls must find its chunks here rather than free and
+// allocate them again.  A call that runs short of memory returns the
+// cache's chunks to the device first (Call::carve).
 namespace {
 std::mutex chunk_mu;
 std::multimap<std::pair<int, size_t>, void *> chunk_cache;   // (device, bytes) -> chunk
 size_t chunk_cached = 0;
-constexpr size_t CHUNK_CAP = (size_t)2 << 30;                 // bytes kept between calls, per process
+
+size_t chunk_cap() {
+    static size_t cap = 0;
+    if (!cap) {
+        size_t free_b = 0, total_b = 0;
+        cap = hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b ? total_b / 4 : (size_t)2 << 30;
+    }
+    return cap;
+}
 }  // namespace
 
 bool chunk_take(size_t bytes, DeviceBuffer &out) {
@@ -229,7 +243,7 @@ void chunk_give(DeviceBuffer &&chunk) {
     int dev = 0;
     if (hipGetDevice(&dev) == hipSuccess) {
         std::lock_guard<std::mutex> lock(chunk_mu);
-        if (chunk_cached + chunk.bytes <= CHUNK_CAP) {
+        if (chunk_cached + chunk.bytes <= chunk_cap()) {
             chunk_cache.emplace(std::make_pair(dev, chunk.bytes), chunk.ptr);
             chunk_cached += chunk.bytes;
             chunk.ptr = nullptr;

@@ -69,6 +69,8 @@ void chunk_drop();
 // small buffers a fold needs cost no hipMalloc once a call of that size
 // has run.  When the call ends its stream is synchronized and the chunks go
 // back to the cache.
+constexpr size_t CHUNK_GROW_MAX = (size_t)1 << 30;   // largest chunk carved for many buffers
+
 struct Call {
     const ef_options *options = nullptr;
     hipStream_t stream = nullptr;
@@ -95,8 +97,12 @@ struct Call {
         n = (n + 255) & ~(size_t)255;
         if (buffers.empty() || used + n > buffers.back().bytes) {
             const size_t last = buffers.empty() ? 0 : buffers.back().bytes;
+            // Chunks double from 4 MiB up to 1 GiB, so a call's many small
+            // buffers take few allocations; a larger request gets a chunk of
+            // its own size, so a call never holds much more than it uses.
             size_t chunk = (size_t)4 << 20;
             if (chunk < 2 * last) chunk = 2 * last;
+            if (chunk > CHUNK_GROW_MAX) chunk = CHUNK_GROW_MAX;
             if (chunk < n) chunk = n;
             buffers.emplace_back();
             if (!chunk_take(n, buffers.back())) {

@@ -36,9 +36,11 @@ def test_dense_contraction_bit_exact_full_grid(cuda, n):
 
 
 # Block edges of dense.rs:68-75 (XB/YB/ZB +-1), batches, small outputs with
-# several z blocks, NULL and absent cells of a, and special values in b.
+# several z blocks, NULL and absent cells of a, and special values in b;
+# (40, 300, 50) takes the one-cell-per-thread kernel over two z chunks of
+# 256, the second ending in a run shorter than 16.
 EDGE_SHAPES = [(255, 3, 257), (257, 5, 63), (64, 64, 64), (3, 4097, 5), (2, 8193, 3),
-               (1, 1, 1), (100, 33, 2049)]
+               (1, 1, 1), (100, 33, 2049), (40, 300, 50)]
 
 
 @pytest.mark.parametrize("nx,nz,ny", EDGE_SHAPES)
