@@ -1461,6 +1461,16 @@ bool squared_difference(const std::vector<ef_expr_op> &e, PipeArgs &pa) {
             pa.sq_mono = 0;
     }
     pa.sq_neg_div = n == 2 && pa.post_code[0] == EF_EXPR_NEG && pa.post_code[1] == EF_EXPR_DIV;
+    pa.sq_div_pow2 = 0;
+    pa.sq_neg_recip = 0.0;
+    if (pa.sq_neg_div && std::isfinite(pa.post_value[1]) && pa.post_value[1] != 0.0) {
+        int ex = 0;
+        const double c = -pa.post_value[1], r = 1.0 / c;
+        if (std::frexp(std::fabs(c), &ex) == 0.5 && std::isnormal(r) && r * c == 1.0) {
+            pa.sq_div_pow2 = 1;
+            pa.sq_neg_recip = r;
+        }
+    }
     pa.sq_swap = e[0].operand == 1 ? 1 : 0;
     pa.sq_q = (int)(e[0].operand == 0 ? e[0].column : e[1].column);
     pa.sq_k = (int)(e[0].operand == 1 ? e[0].column : e[1].column);

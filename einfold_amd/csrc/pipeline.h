@@ -33,8 +33,13 @@ struct PipeArgs {
     // of the square (every literal finite, no product or quotient by 0), so
     // a tile's extreme scores are those of its extreme squares.
     // sq_neg_div: the operations are exactly a negation then a division by
-    // post_value[1], which is one division by -post_value[1] (the same bits).
-    int sq_mono, sq_neg_div;
+    // post_value[1], which is one division by -post_value[1] (the same bits);
+    // sq_div_pow2: that divisor is a power of two whose reciprocal is a
+    // normal double, sq_neg_recip = 1 / -post_value[1] exactly, and the
+    // division is that product (the same real quotient, rounded once: the
+    // same bits).
+    int sq_mono, sq_neg_div, sq_div_pow2;
+    double sq_neg_recip;
     const ef_expr_op *weight;       // device; n_weight > 0: Weight::Of
     int n_weight;
     uint32_t scale_kind;

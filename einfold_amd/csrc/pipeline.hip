@@ -270,7 +270,7 @@ __device__ __forceinline__ double score_of(const PipeArgs &p, int64_t x, int64_t
 
 // The operations after the square.
 __device__ __forceinline__ double sq_post(const PipeArgs &p, double t) {
-    if (p.sq_neg_div) return t / -p.post_value[1];   // -(t) / c, exactly
+    if (p.sq_neg_div) return p.sq_div_pow2 ? t * p.sq_neg_recip : t / -p.post_value[1];   // -(t) / c, exactly
     for (int i = 0; i < p.n_post; i++) {
         const double c = p.post_value[i];
         switch (p.post_code[i]) {
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(64) void pipeline_rows_kernel(PipeArgs p) {
     row_finish(p, st, a, special_value);
 }
 
-// Part `t % parts` of row `t / parts`: a contiguous run of the row's J
+// Part `t / nx` of row `t % nx`: a contiguous run of the row's J
 // tiles (the row's tiles split into `parts` nearly equal runs), its state
 // and sums written to the part's slots.
 template <int FORM>
@@ -560,8 +560,11 @@ __global__ __launch_bounds__(256) void pipeline_parts_kernel(PipeArgs p, int par
                                                              uint8_t *psv) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= p.nx * parts) return;
-    const int64_t x = t / parts;
-    const int part = (int)(t % parts);
+    // Neighbouring threads take the same part of neighbouring rows: their
+    // runs cover the same keys (for rows that see the same keys), so a
+    // wave's key loads read one line instead of one per lane.
+    const int64_t x = t % p.nx;
+    const int part = (int)(t / p.nx);
     double *a = pa + t * p.ny;
     uint8_t *special_value = psv + t * p.ny;
     for (int64_t y = 0; y < p.ny; y++) {
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(256) void pipeline_merge_kernel(PipeArgs p, int par
         special_value[y] = 0;
     }
     for (int part = 0; part < parts; part++) {
-        const int64_t t = x * parts + part;
+        const int64_t t = part * p.nx + x;
         const RowState s = states[t];
         const double *ap = pa + t * ny;
         const uint8_t *sp = psv + t * ny;
