@@ -76,10 +76,14 @@ ef_status gustavson_heavy(const GustavsonArgs &g, const uint32_t *heavy_host, co
                           const uint64_t *run_off_host, const uint64_t *run_off_dev, uint32_t *n_runs, hipStream_t s);
 ef_status gustavson(const GustavsonArgs &g, int blocks, hipStream_t s);
 ef_status widen_counts(const uint32_t *in, int64_t n, uint64_t *out, hipStream_t s);
-ef_status emit_groups(int64_t nx, const uint64_t *bound_off, const uint64_t *out_off, const uint32_t *g_count,
-                      const uint32_t *g_y, const double *g_sum, const uint8_t *g_valued, const uint32_t *x_first,
-                      const uint32_t *y_first, const GroupKeys &keys, double *values, uint8_t *valid,
-                      hipStream_t s);
+// Device bytes emit_groups needs in `scratch` for ny y ranks.
+size_t emit_scratch_bytes(int64_t ny);
+// The groups as output rows (keys, sums, validity); ny: the y ranks that
+// exist (y_first[0 .. ny - 1] valid).
+ef_status emit_groups(int64_t nx, int64_t ny, const uint64_t *bound_off, const uint64_t *out_off,
+                      const uint32_t *g_count, const uint32_t *g_y, const double *g_sum, const uint8_t *g_valued,
+                      const uint32_t *x_first, const uint32_t *y_first, const GroupKeys &keys, void *scratch,
+                      double *values, uint8_t *valid, hipStream_t s);
 ef_status domain_lookup(int64_t n, const uint8_t *matches, const uint32_t *x_rank, const uint32_t *y_rank, int64_t nx,
                         const uint64_t *bound_off, const uint32_t *g_count, const uint32_t *g_y, const double *g_sum,
                         const uint8_t *g_valued, int has_missing, double missing, double *values, uint8_t *valid,

@@ -576,15 +576,29 @@ __device__ __forceinline__ void copy_key(const GroupKeys &k, int i, int64_t src,
     k.dst_valid[i][dst] = k.src_valid[i][src];
 }
 
+// b's output keys by y rank: key i's value and validity at the first row
+// of each y rank, rank r at index r (one thread per rank and key).
+__global__ __launch_bounds__(256) void y_keys_kernel(int64_t ny, const uint32_t *y_first, GroupKeys src,
+                                                     GroupKeys by_rank) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ny) return;
+    for (int i = 0; i < src.n; i++) {
+        if (src.from_a[i]) continue;
+        GroupKeys k = src;
+        k.dst[i] = const_cast<void *>(by_rank.src[i]);
+        k.dst_valid[i] = const_cast<uint8_t *>(by_rank.src_valid[i]);
+        copy_key(k, i, y_first[r], r);
+    }
+}
+
 // The output's rows, one wave per x: each group's keys taken from the
-// first row of its x rank (a's outputs) or y rank (b's), and its sum or
-// NULL (discriminate.rs:586-610).
+// first row of its x rank (a's outputs) or, already gathered by rank, its y
+// rank (b's), and its sum or NULL (discriminate.rs:586-610).
 __global__ __launch_bounds__(256) void emit_groups_kernel(int64_t nx, const uint64_t *bound_off,
                                                           const uint64_t *out_off, const uint32_t *g_count,
                                                           const uint32_t *g_y, const double *g_sum,
                                                           const uint8_t *g_valued, const uint32_t *x_first,
-                                                          const uint32_t *y_first, GroupKeys keys, double *values,
-                                                          uint8_t *valid) {
+                                                          GroupKeys keys, double *values, uint8_t *valid) {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * 4;
     for (int64_t x = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); x < nx; x += waves) {
@@ -594,7 +608,7 @@ __global__ __launch_bounds__(256) void emit_groups_kernel(int64_t nx, const uint
         for (uint32_t t = lane; t < n; t += 64) {
             const uint32_t y = g_y[src + t];
             const int64_t o = (int64_t)(dst + t);
-            for (int i = 0; i < keys.n; i++) copy_key(keys, i, keys.from_a[i] ? xr : y_first[y], o);
+            for (int i = 0; i < keys.n; i++) copy_key(keys, i, keys.from_a[i] ? xr : y, o);
             values[o] = g_sum[src + t];
             valid[o] = g_valued[src + t];
         }
@@ -856,14 +870,34 @@ ef_status widen_counts(const uint32_t *in, int64_t n, uint64_t *out, hipStream_t
     return EF_OK;
 }
 
-ef_status emit_groups(int64_t nx, const uint64_t *bound_off, const uint64_t *out_off, const uint32_t *g_count,
-                      const uint32_t *g_y, const double *g_sum, const uint8_t *g_valued, const uint32_t *x_first,
-                      const uint32_t *y_first, const GroupKeys &keys, double *values, uint8_t *valid,
-                      hipStream_t s) {
+size_t emit_scratch_bytes(int64_t ny) { return 8 * ((((size_t)(ny > 0 ? ny : 1) * 8 + 255) & ~(size_t)255) + (((size_t)(ny > 0 ? ny : 1) + 255) & ~(size_t)255)); }
+
+ef_status emit_groups(int64_t nx, int64_t ny, const uint64_t *bound_off, const uint64_t *out_off,
+                      const uint32_t *g_count, const uint32_t *g_y, const double *g_sum, const uint8_t *g_valued,
+                      const uint32_t *x_first, const uint32_t *y_first, const GroupKeys &keys, void *scratch,
+                      double *values, uint8_t *valid, hipStream_t s) {
     if (nx == 0) return EF_OK;
+    // b's keys gathered once per y rank (ny values each), so that the
+    // emission reads a small table instead of scattered rows of the
+    // original columns for every group.
+    GroupKeys emit = keys, by_rank = keys;
+    char *at = static_cast<char *>(scratch);
+    bool any_b = false;
+    for (int i = 0; i < keys.n; i++) {
+        if (keys.from_a[i]) continue;
+        any_b = true;
+        by_rank.src[i] = at;
+        at += ((size_t)ny * 8 + 255) & ~(size_t)255;
+        by_rank.src_valid[i] = reinterpret_cast<uint8_t *>(at);
+        at += ((size_t)ny + 255) & ~(size_t)255;
+        emit.src[i] = by_rank.src[i];
+        emit.src_valid[i] = by_rank.src_valid[i];
+    }
+    if (any_b && ny > 0)
+        hipLaunchKernelGGL(y_keys_kernel, dim3(blocks_for(ny, 256)), dim3(256), 0, s, ny, y_first, keys, by_rank);
     const int64_t blocks = (nx + 3) / 4 < 4096 ? (nx + 3) / 4 : 4096;
     hipLaunchKernelGGL(emit_groups_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nx, bound_off, out_off, g_count,
-                       g_y, g_sum, g_valued, x_first, y_first, keys, values, valid);
+                       g_y, g_sum, g_valued, x_first, emit, values, valid);
     EF_HIP(hipGetLastError());
     return EF_OK;
 }

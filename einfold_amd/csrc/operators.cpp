@@ -1333,8 +1333,10 @@ ef_status fold_discriminate_impl(const FoldModel &m, const Relation &a, const Re
         uint8_t *valid;
         EF_TRY(c.alloc(vals, total));
         EF_TRY(c.alloc(valid, total));
-        EF_TRY(emit_groups(nx, bound_off, out_off, g_count, g_y, g_sum, g_valued, x_first, y_first, keys, vals, valid,
-                           c.stream));
+        uint8_t *emit_scratch;
+        EF_TRY(c.alloc(emit_scratch, emit_scratch_bytes((int64_t)ky)));
+        EF_TRY(emit_groups(nx, (int64_t)ky, bound_off, out_off, g_count, g_y, g_sum, g_valued, x_first, y_first, keys,
+                           emit_scratch, vals, valid, c.stream));
         std::vector<OutColumn> cols;
         for (size_t i = 0; i < m.outputs.size(); i++) {
             OutColumn k;
