@@ -204,38 +204,63 @@ void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, const int *wg
     }
     // Workgroup g computes items wg_off[g] .. wg_off[g + 1] - 1 (the host
     // balanced the CUs' shares).
-    for (int it = wg_off[blockIdx.x]; it < wg_off[blockIdx.x + 1]; it++) {
+    // The next item's first k step is loaded into registers during this
+    // item's last step (and its epilogue), so consecutive items' loads
+    // overlap: most items here have few k steps.
+    u32x4 pre[N];
+    const int it_end = wg_off[blockIdx.x + 1];
+    auto gload = [&](const E *A_, const E *B_, int64_t aplane_, int64_t bplane_, int64_t kp_, int64_t k0_)
+        __attribute__((always_inline)) {
+#pragma unroll
+        for (int i_ = 0; i_ < N; i_++) {
+            const E *src_ = i_ < AN ? A_ + cpl[i_] * aplane_ : B_ + cpl[i_] * bplane_;
+            pre[i_] = *reinterpret_cast<const u32x4 *>(src_ + (int64_t)crow_[i_] * kp_ + k0_ + ccol[i_]);
+        }
+    };
+    // Item `it`'s operand bases, plane strides and padded k.
+    struct Ops { const E *A, *B; int64_t aplane, bplane, kp; };
+    auto ops_of = [&](const Item &item, const GemmDesc &d) __attribute__((always_inline)) {
+        Ops o;
+        o.A = ops + d.a_off + (int64_t)item.tm * BM * d.kp;
+        o.B = ops + d.b_off + (int64_t)item.tn * BN * d.kp;
+        o.aplane = d.mp * d.kp;
+        o.bplane = d.np * d.kp;
+        o.kp = d.kp;
+        return o;
+    };
+    if (wg_off[blockIdx.x] < it_end) {
+        const Item first = items[wg_off[blockIdx.x]];
+        const Ops o = ops_of(first, descs[first.problem]);
+        if (o.kp > 0) gload(o.A, o.B, o.aplane, o.bplane, o.kp, 0);
+    }
+    for (int it = wg_off[blockIdx.x]; it < it_end; it++) {
         const Item item = items[it];
         const GemmDesc d = descs[item.problem];
         const int64_t m0 = (int64_t)item.tm * BM, n0 = (int64_t)item.tn * BN;
-        const E *A = ops + d.a_off + m0 * d.kp;
-        const E *B = ops + d.b_off + n0 * d.kp;
-        const int64_t aplane = d.mp * d.kp, bplane = d.np * d.kp;
+        const Ops o = ops_of(item, d);
+        const bool has_next = it + 1 < it_end;
+        Ops no{};
+        if (has_next) {
+            const Item nitem = items[it + 1];
+            no = ops_of(nitem, descs[nitem.problem]);
+        }
         AccV acc[IM][JN];
 #pragma unroll
         for (int i = 0; i < IM; i++)
 #pragma unroll
             for (int j = 0; j < JN; j++) acc[i][j] = AccV{0, 0, 0, 0};
-        u32x4 pre[N];
-#define EF_GLOAD(K0)                                                                                  \
-    _Pragma("unroll") for (int i_ = 0; i_ < N; i_++) {                                               \
-        const E *src_ = i_ < AN ? A + cpl[i_] * aplane : B + cpl[i_] * bplane;                        \
-        pre[i_] = *reinterpret_cast<const u32x4 *>(src_ + (int64_t)crow_[i_] * d.kp + (K0) + ccol[i_]); \
-    }
 #define EF_LSTORE(BUFP)                                                                               \
     _Pragma("unroll") for (int i_ = 0; i_ < N; i_++) {                                               \
         E *dst_ = (BUFP) + (i_ < AN ? cpl[i_] * APLANE : PL * APLANE + cpl[i_] * BPLANE);             \
         *reinterpret_cast<u32x4 *>(dst_ + crow_[i_] * PITCH + ccol[i_]) = pre[i_];                    \
     }
         const int64_t nk = d.kp / KSTEP;   // 0 for k = 0: no loads, C = 0
-        if (nk > 0) {
-            EF_GLOAD(0);
-            EF_LSTORE(lds);
-        }
+        if (nk > 0) EF_LSTORE(lds);        // step 0, loaded before this item began
         __syncthreads();
         for (int64_t ks = 0; ks < nk; ks++) {
             const int cur = (int)(ks & 1);
-            if (ks + 1 < nk) EF_GLOAD((ks + 1) * KSTEP);
+            if (ks + 1 < nk) gload(o.A, o.B, o.aplane, o.bplane, o.kp, (ks + 1) * KSTEP);
+            else if (has_next && no.kp > 0) gload(no.A, no.B, no.aplane, no.bplane, no.kp, 0);
             const E *as = lds + cur * BUF;
             const E *bs = as + PL * APLANE;
 #pragma unroll
@@ -261,7 +286,7 @@ void grouped_gemm_kernel(const GemmDesc *descs, const Item *items, const int *wg
             if (ks + 1 < nk) EF_LSTORE(lds + (cur ^ 1) * BUF);
             __syncthreads();
         }
-#undef EF_GLOAD
+        if (nk == 0 && has_next && no.kp > 0) gload(no.A, no.B, no.aplane, no.bplane, no.kp, 0);
 #undef EF_LSTORE
         // Lane holds C[m0 + wr*WM + i*16 + crow(g, r)][n0 + wc*WN + j*16 + c].
 #pragma unroll
