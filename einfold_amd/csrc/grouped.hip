@@ -454,10 +454,11 @@ void grouped_gemm_dma_kernel(const GemmDesc *descs, const Item *items, const int
 
 size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Tile shapes: config 0 is 128 x 128, 1 is 64 x 64, 2 the large shape of
-// the BF16 precisions (256 x 256 for BF16; 256 x 128 with 32-wide k steps
-// for BF16x3, whose two planes would not fit LDS twice at 256 x 256).
-bool config_valid(uint32_t precision, int config) { return config < 2 || precision != EF_PRECISION_FP32; }
+// Tile shapes: config 0 is 128 x 128, 1 is 64 x 64, 2 the large shape
+// (256 x 256 for BF16; 256 x 128 with 32-wide k steps for BF16x3, whose two
+// planes would not fit LDS twice at 256 x 256, and for FP32, whose
+// 128 x 128 per wave at 256 x 256 would spill registers).
+bool config_valid(uint32_t precision, int config) { return config <= 2; }
 int tile_m(int config) { return config == 0 ? 128 : config == 1 ? 64 : 256; }
 int tile_n(uint32_t precision, int config) {
     return config == 0 ? 128 : config == 1 ? 64 : (precision == EF_PRECISION_BF16 ? 256 : 128);
@@ -514,6 +515,7 @@ template <int P> int gemm_slots_of(int config) {
     if (config == 1) return occupancy(grouped_gemm_kernel<P, 64, 64, 2, 2, 64>, 256);
     if constexpr (P == PREC_BF16) return occupancy(grouped_gemm_dma_kernel<P, 256, 256, 2, 4, 64>, 512);
     if constexpr (P == PREC_BF16X3) return occupancy(grouped_gemm_kernel<P, 256, 128, 4, 2, 32>, 512);
+    if constexpr (P == PREC_F32) return occupancy(grouped_gemm_kernel<P, 256, 128, 2, 2, 32>, 256);
     return 1;
 }
 
@@ -644,8 +646,9 @@ ef_status launch_steps(const GroupedPlan &p, hipStream_t s) {
         hipLaunchKernelGGL((grouped_gemm_kernel<P, 256, 128, 4, 2, 32>), dim3((unsigned)p.grid), dim3(512), 0, s,
                            p.descs, p.items, p.wg_off, (const E *)p.ops);
     } else {
-        set_error("no large tile shape for FP32 products");
-        return EF_INVALID;
+        // FP32: 256 x 128, four waves of 128 x 64, 32-wide k steps
+        hipLaunchKernelGGL((grouped_gemm_kernel<P, 256, 128, 2, 2, 32>), dim3((unsigned)p.grid), dim3(256), 0, s,
+                           p.descs, p.items, p.wg_off, (const E *)p.ops);
     }
     EF_HIP(hipGetLastError());
     return EF_OK;
