@@ -559,6 +559,37 @@ def sparse_product_line(with_cpu: bool):
         if not same:
             raise SystemExit("sparse product: device groups or sums differ from the oracle's")
         line["bit_exact_vs_oracle"] = True
+        # The host plan this path replaces (a hash join on j, then a hash
+        # aggregate by (i, k)), stood in for by pyarrow's Acero engine on this
+        # host's cores -- not DataFusion -- at 20k rows per side (its join
+        # materializes every joined pair), beside the device at that size.
+        small = 20_000
+        sa = pa.table({"i": a.column(0).slice(0, small), "j": a.column(1).slice(0, small),
+                       "va": pc.cast(a.column(2).slice(0, small), pa.float64())})
+        sb = pa.table({"j": b.column(0).slice(0, small), "k": b.column(1).slice(0, small),
+                       "vb": pc.cast(b.column(2).slice(0, small), pa.float64())})
+        plans = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            jn = sa.join(sb, keys="j", join_type="inner")
+            jn = jn.append_column("p", pc.multiply(jn.column("va"), jn.column("vb")))
+            g = jn.group_by(["i", "k"]).aggregate([("p", "sum")])
+            plans.append(time.perf_counter() - t0)
+            if plans[-1] > 30.0:   # time cap: one run is enough
+                break
+        dev_small = []
+        sa32, sb32 = a.slice(0, small), b.slice(0, small)
+        kernels.fold(f, sa32, sb32, options=opts)
+        for _ in range(3):
+            t0 = time.perf_counter()
+            got = kernels.fold(f, sa32, sb32, options=opts)
+            dev_small.append(time.perf_counter() - t0)
+        assert sum(x.num_rows for x in got.batches) == g.num_rows
+        line["hash_plan_20k"] = {"joined_rows": jn.num_rows, "groups": g.num_rows,
+                                 "hash_plan_ms": round(float(np.median(plans)) * 1e3, 1),
+                                 "device_arrow_to_arrow_ms": round(float(np.median(dev_small)) * 1e3, 2),
+                                 "hash_plan": f"pyarrow Acero hash join + hash aggregate, {pa.cpu_count()} threads "
+                                              "(30 s cap per run): a stand-in for DataFusion's plan, not DataFusion"}
     return line
 
 
@@ -692,6 +723,18 @@ def ranking_line(dev, with_cpu: bool = False):
     line = {"workload": "ranking of 8M UInt64 join keys (Zipf 1.1 over 1M ids): hash dedup, distinct words ordered by bucket", "keys": n,
             "distinct": int(k.value), "ms": round(ms, 4), "gb_per_s": round(gbs, 1),
             "frac_of_hbm_peak": round(gbs / 8000.0, 4)}
+    # Config 2's group keys: i (and k alike), 4M UInt64 hashes of uniform
+    # ids over 64k.
+    gi = torch.from_numpy(data.hash64(data.zipf_ids(5, 3, rows, 1 << 16, 0.0), 0x11).view(np.int64)).to(dev)
+    ki = ctypes.c_size_t(0)
+    rank_i = torch.empty(rows, dtype=torch.int32, device=dev)
+    first_i = torch.empty(rows, dtype=torch.int32, device=dev)
+    run_i = lambda: check(lib().ef_rank_u64(P(gi), rows, P(rank_i), P(first_i), ctypes.byref(ki), P(scratch), stream))
+    ms_i = _events(run_i, 10)
+    gbs_i = 12.0 * rows / (ms_i * 1e-3) / 1e9
+    line["group_keys"] = {"workload": "ranking of 4M UInt64 group keys (uniform over 64k ids), config 2's i and k",
+                          "keys": rows, "distinct": int(ki.value), "ms": round(ms_i, 4), "gb_per_s": round(gbs_i, 1),
+                          "frac_of_hbm_peak": round(gbs_i / 8000.0, 4)}
     if with_cpu:
         import oracle
         host = np.concatenate([a_j, b_j])
