@@ -50,18 +50,41 @@ __global__ __launch_bounds__(256) void dense_accumulate_kernel(const double *__r
         for (int c = 0; c < RN; c++) acc[r][c] = 0.0;
     const double *bw = b + w * nz * ny;
     for (int64_t z0 = 0; z0 < nz; z0 += ZC) {
-        for (int i = tid; i < TM * ZC; i += 256) {
-            const int r = i / ZC, zz = i % ZC;
+        static_assert(TM * ZC % 256 == 0 && ZC * TN % 256 == 0, "whole staging rounds");
+        constexpr int NA = TM * ZC / 256, NB = ZC * TN / 256;
+        // Every load of the step issued before any LDS store, each from a
+        // clamped cell (x0 and z0 are inside), then 0 outside: a load behind
+        // `in &&` becomes a branch and a wait per element.
+        double ra[NA], rb[NB];
+        bool rv[NA];
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+            const int i = tid + 256 * k, r = i / ZC, zz = i % ZC;
             const int64_t x = x0 + r, z = z0 + zz;
             const bool in = x < xend && x < nx && z < nz;
-            const bool v = in && held[x * nz + z] == VALUED;
-            as[r][zz] = v ? a[x * nz + z] : 0.0;
-            fs[r][zz] = v ? 1 : 0;
+            const int64_t xc = in ? x : x0, zc = in ? z : z0;
+            ra[k] = a[xc * nz + zc];
+            const uint32_t hv = held[xc * nz + zc];
+            rv[k] = in && hv == VALUED;
         }
-        for (int i = tid; i < ZC * TN; i += 256) {
-            const int zz = i / TN, cc = i % TN;
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int i = tid + 256 * k, zz = i / TN, cc = i % TN;
             const int64_t z = z0 + zz, y = y0 + cc;
-            bs[zz][cc] = (z < nz && y < ny) ? bw[z * ny + y] : 0.0;
+            const bool inb = z < nz && y < ny;
+            const double bv = bw[(inb ? z : z0) * ny + (inb ? y : y0)];
+            rb[k] = inb ? bv : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+            const int i = tid + 256 * k;
+            as[i / ZC][i % ZC] = rv[k] ? ra[k] : 0.0;
+            fs[i / ZC][i % ZC] = rv[k] ? 1 : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int i = tid + 256 * k;
+            bs[i / TN][i % TN] = rb[k];
         }
         __syncthreads();
         const int zn = nz - z0 < ZC ? (int)(nz - z0) : ZC;
@@ -134,7 +157,10 @@ __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__res
             // Unconditional loads from a clamped cell, then 0 outside.
             const int64_t xc = in ? x : x0, zcl = in ? z : 0;
             const double av = a[xc * nz + zcl];
-            const bool v = in && held[xc * nz + zcl] == VALUED;
+            // The flag read unconditionally too: behind `in &&` it becomes a
+            // branch with a wait, and the 16 loads run one after another.
+            const uint32_t hv = held[xc * nz + zcl];
+            const bool v = in && hv == VALUED;
             ra[k] = v ? av : 0.0;
             rf[k] = v ? 1 : 0;
             const int zb = i / T, cc = i % T;
