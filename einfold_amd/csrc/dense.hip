@@ -9,6 +9,7 @@
 // but each cell's chain is sequential, so the bits are the reference's.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "common.h"
 #include "mfma.h"
@@ -128,7 +129,9 @@ __global__ __launch_bounds__(256) void dense_accumulate_kernel(const double *__r
 // loaded into registers while this one is summed.  A chunk's chain of 256
 // sums lasts about one memory latency, so the next chunk's loads are hidden
 // behind it, and 256^3 takes one exposed load in all.  Each cell's chain is
-// the same z-ordered sequence of roundings, so the same bits.
+// the same z-ordered sequence of roundings, so the same bits.  PRED: the
+// caller does not vouch that b is finite, so each chunk checks its b slab
+// and skips non-valued cells by their flags where the slab is not.
 template <bool PRED>
 __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__restrict__ a,
                                                              const uint32_t *__restrict__ held,
@@ -139,7 +142,8 @@ __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__res
     __shared__ double as[T][ZT + 1];
     __shared__ uint8_t fs[T][ZT + 1];
     __shared__ double bs[ZT][T];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    __shared__ int wave_finite[4];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4, lane = tid & 63;
     const int64_t w = blockIdx.z;
     const int64_t x0 = w * per_batch + (int64_t)blockIdx.x * T;
     const int64_t xend = w * per_batch + per_batch < nx ? w * per_batch + per_batch : nx;
@@ -171,18 +175,12 @@ __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__res
         }
     };
     double acc = 0.0;
-    if (nz > 0) fetch(0);
-    for (int64_t z0 = 0; z0 < nz; z0 += ZT) {
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int i = tid + 256 * k;
-            as[i / ZT][i % ZT] = ra[k];
-            fs[i / ZT][i % ZT] = rf[k];
-            bs[i / T][i % T] = rb[k];
-        }
-        __syncthreads();
-        if (z0 + ZT < nz) fetch(z0 + ZT);
-        const int zn = nz - z0 < ZT ? (int)(nz - z0) : ZT;
+    // One chunk's sums.  FLAGS: skip the cells `held` does not mark valued
+    // with a select on the product; without, their zero factors give +-0
+    // products, which leave a sum from +0.0 as it is (such a sum is never
+    // -0.0) when b is finite: the same bits either way.
+    auto sum_chunk = [&](auto flags, int zn) {
+        constexpr bool FLAGS = decltype(flags)::value;
         // 16 z at a time: every LDS read of the run issued before the chain
         // of sums that uses them, so the chain waits on one read latency per
         // run instead of one per z.
@@ -194,7 +192,7 @@ __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__res
             for (int j = 0; j < 16; j++) {
                 av[j] = as[ty][zz0 + j];
                 bv[j] = bs[zz0 + j][tx];
-                fv[j] = !PRED || fs[ty][zz0 + j];
+                fv[j] = !FLAGS || fs[ty][zz0 + j];
             }
             // A skipped cell adds -0.0, which leaves every sum as it is
             // (x + -0.0 == x, signed zeros included): the select is taken on
@@ -202,7 +200,31 @@ __global__ __launch_bounds__(256) void dense_accumulate_thin(const double *__res
 #pragma unroll
             for (int j = 0; j < 16; j++) acc = acc + (fv[j] ? av[j] * bv[j] : -0.0);
         }
-        for (int zz = zz0; zz < zn; zz++) acc = acc + ((!PRED || fs[ty][zz]) ? as[ty][zz] * bs[zz][tx] : -0.0);
+        for (int zz = zz0; zz < zn; zz++) acc = acc + ((!FLAGS || fs[ty][zz]) ? as[ty][zz] * bs[zz][tx] : -0.0);
+    };
+    if (nz > 0) fetch(0);
+    for (int64_t z0 = 0; z0 < nz; z0 += ZT) {
+        int finite = 1;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + 256 * k;
+            as[i / ZT][i % ZT] = ra[k];
+            fs[i / ZT][i % ZT] = rf[k];
+            bs[i / T][i % T] = rb[k];
+            finite &= isfinite(rb[k]) ? 1 : 0;
+        }
+        // The flags are needed only where the workgroup's b slab of this
+        // chunk holds an infinite or NaN value: a vote per wave, read by all
+        // after the barrier that also publishes the tiles (each wave's slot
+        // is written again only after the chunk's closing barrier).
+        const int wave_all = __all(finite);   // the vote of every lane, outside the branch
+        if (PRED && lane == 0) wave_finite[tid >> 6] = wave_all;
+        __syncthreads();
+        const bool flags = PRED && !(wave_finite[0] && wave_finite[1] && wave_finite[2] && wave_finite[3]);
+        if (z0 + ZT < nz) fetch(z0 + ZT);
+        const int zn = nz - z0 < ZT ? (int)(nz - z0) : ZT;
+        if (flags) sum_chunk(std::true_type{}, zn);
+        else sum_chunk(std::false_type{}, zn);
         __syncthreads();
     }
     const int64_t x = x0 + ty, y = y0 + tx;

@@ -911,7 +911,11 @@ ef_status fold_dense_impl(const FoldModel &m, const Relation &a, const Relation 
         // order, within 2 (n - 1) u sum|terms| of the exact one.
         EF_TRY(launch_dense_product_f64(am, bm, out, nw, pb, ny, nz, c.stream));
     } else {
-        EF_TRY(launch_dense_accumulate_f64(am, held, bm, nx, nz, ny, pb, 1, out, c.stream));
+        // The predicated kernel skips the cells `held` does not mark valued
+        // with a select.  When every b is finite, those cells' zero factors
+        // give +-0 products instead, and adding +-0 leaves a sum from +0.0 as
+        // it is (such a sum is never -0.0): the same bits, without the flags.
+        EF_TRY(launch_dense_accumulate_f64(am, held, bm, nx, nz, ny, pb, pf.b_not_finite ? 1 : 0, out, c.stream));
     }
 
     return finish_fold();

@@ -99,6 +99,28 @@ def test_dense_nulls_and_missing_rows_of_a(cuda):
     assert t.column(2).null_count >= 0
 
 
+@pytest.mark.parametrize("nx,nz,ny", [(300, 200, 100), (736, 20, 736), (2048, 8, 2048)])
+@pytest.mark.parametrize("b_infinite", [False, True])
+def test_dense_skipped_cells_with_finite_and_infinite_b(cuda, nx, nz, ny, b_infinite):
+    """Cells of a that are absent or NULL add nothing to their sums
+    (dense.rs:706-770).  With every b finite the contraction multiplies them
+    as zeros (a +-0 product leaves a sum from +0.0 as it is); with an
+    infinite or NaN b it skips them by their flags, since 0 * inf would be
+    NaN.  Both give the reference's bits, on each contraction kernel: one
+    cell per thread, 32 x 32 and 64 x 64 tiles."""
+    rng = np.random.default_rng(nx + nz + ny + b_infinite)
+    a = _grid(rng, (nx, nz), ["i", "j"], null_p=0.1, drop=0.2)
+    b = _grid(rng, (nz, ny), ["j", "k"])
+    if b_infinite:
+        v = b.column("v").to_numpy().copy()
+        v[rng.choice(len(v), 5, replace=False)] = [np.inf, -np.inf, np.nan, np.inf, -np.inf]
+        b = b.set_column(2, "v", pa.array(v))
+    got, want = _both_dense(_grid_fold(), a, b)
+    assert_same(got, want)
+    if b_infinite:
+        assert np.isnan(pa.Table.from_batches(want).column(2).to_numpy(zero_copy_only=False)).any()
+
+
 def test_dense_declines_holes_in_b_without_a_zero_default(cuda):
     rng = np.random.default_rng(4)
     a = _grid(rng, (20, 20), ["i", "j"])
