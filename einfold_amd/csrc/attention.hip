@@ -515,11 +515,19 @@ void attn_fwd_kernel(FwdArgs a) {
             const E *ks = lds + cur * F::BUF;
             const E *vt = ks + F::PL * F::KPLANE;
 
+            // FP32 accumulators: the scores arrive less the reference max
+            // (they start at -base, base = m, or 0 before any key is seen),
+            // so the weights need no subtraction unless the reference moves.
+            // FP64 (base 0) keeps the subtraction: its registers are full.
+            constexpr bool FOLD = P != PREC_F64;
+            S base[2];
             AccV s[2][4];
 #pragma unroll
-            for (int qb = 0; qb < 2; qb++)
+            for (int qb = 0; qb < 2; qb++) {
+                base[qb] = FOLD && m[qb] != -INFINITY ? m[qb] : S(0);
 #pragma unroll
-                for (int kb = 0; kb < 4; kb++) s[qb][kb] = AccV{0, 0, 0, 0};
+                for (int kb = 0; kb < 4; kb++) s[qb][kb] = AccV{-base[qb], -base[qb], -base[qb], -base[qb]};
+            }
 #pragma unroll
             for (int kc = 0; kc < KC; kc++) {
 #pragma unroll
@@ -568,37 +576,50 @@ void attn_fwd_kernel(FwdArgs a) {
             // The weights are taken against a reference max m that moves
             // only when a column's tile max passes it by more than TAU (all
             // weights then stay below 2^TAU); mt keeps the true max.  When
-            // no column of the wave moves, o and l need no rescaling.
+            // no column of the wave moves, o and l need no rescaling (and,
+            // with FOLD, the scores are the weights' exponents as they are).
+            // tmax is relative to base; m - base is 0, or -inf before any key
+            // (FOLD), else m.
             S tmax[2];
             bool move = false;
 #pragma unroll
             for (int qb = 0; qb < 2; qb++) {
                 tmax[qb] = wave_max(fmax(fmax(mk[qb][0], mk[qb][1]), fmax(mk[qb][2], mk[qb][3])));
-                mt[qb] = fmax(mt[qb], tmax[qb]);
-                move |= tmax[qb] > m[qb] + S(TAU);
+                mt[qb] = fmax(mt[qb], tmax[qb] + base[qb]);
+                move |= tmax[qb] > (m[qb] - base[qb]) + S(TAU);
             }
             if (__any(move)) {
 #pragma unroll
                 for (int qb = 0; qb < 2; qb++) {
-                    const S mnew = tmax[qb] > m[qb] + S(TAU) ? tmax[qb] : m[qb];
+                    const S mnew = tmax[qb] > (m[qb] - base[qb]) + S(TAU) ? tmax[qb] + base[qb] : m[qb];
+                    const S mnew_sub = mnew == -INFINITY ? S(0) : mnew;
                     // 1 when unmoved, 0 when m was -inf (also when it stays so)
-                    const S alpha = fast_exp2(m[qb] - (mnew == -INFINITY ? S(0) : mnew));
+                    const S alpha = fast_exp2(m[qb] - mnew_sub);
                     m[qb] = mnew;
                     l[qb] *= alpha;
 #pragma unroll
                     for (int db = 0; db < DB; db++)
 #pragma unroll
                         for (int i = 0; i < 4; i++) o[qb][db][i] *= alpha;
+                    if constexpr (FOLD) {
+                        // the exponents against the new reference (0 if unmoved)
+                        const S sh = mnew_sub - base[qb];
+#pragma unroll
+                        for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+                            for (int i = 0; i < 4; i++) s[qb][kb][i] -= sh;
+                    }
                 }
             }
 #pragma unroll
             for (int qb = 0; qb < 2; qb++) {
-                const S msub = m[qb] == -INFINITY ? S(0) : m[qb];   // no key seen yet: weights 0
+                // no key seen yet: weights exp2(-inf) = 0
+                const S off = FOLD || m[qb] == -INFINITY ? S(0) : m[qb];
                 S rk[4];
 #pragma unroll
                 for (int kb = 0; kb < 4; kb++) {
 #pragma unroll
-                    for (int i = 0; i < 4; i++) s[qb][kb][i] = fast_exp2(s[qb][kb][i] - msub);
+                    for (int i = 0; i < 4; i++) s[qb][kb][i] = fast_exp2(FOLD ? s[qb][kb][i] : s[qb][kb][i] - off);
                     rk[kb] = (s[qb][kb][0] + s[qb][kb][1]) + (s[qb][kb][2] + s[qb][kb][3]);
                 }
                 l[qb] += group_sum((rk[0] + rk[1]) + (rk[2] + rk[3]));
