@@ -33,7 +33,7 @@ PEAK_FP32 = 157.3e12
 # HBM bytes one launch of the dK/dV kernel (BF16x3, d = 64) moves at this
 # workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the factors a
 # 256 MiB device copy reads back (profiles/attention_causal_h8_t4096_d64_hbm_traffic.md).
-DKDV_TRAFFIC_BYTES = (29297.3 * 2.0 + 32768.0) * 1024
+DKDV_TRAFFIC_BYTES = (29260.7 * 2.0 + 32768.0) * 1024
 
 
 def pairs_per_head(t: int) -> int:
