@@ -76,9 +76,9 @@ struct Call {
     hipStream_t stream = nullptr;
     std::deque<DeviceBuffer> buffers;   // the chunks
     size_t used = 0;                    // bytes carved from the last chunk
-    bool zero_run = false, zero_split = false;
-    size_t zero_chunk = 0;
-    std::vector<std::pair<void *, size_t>> pending_zeros;
+    bool zero_run = false;
+    struct PendingZero { char *p; size_t bytes, chunk; };
+    std::vector<PendingZero> pending_zeros;
 
     explicit Call(const ef_options *o) : options(o), stream(as_stream(o ? o->stream : nullptr)) {}
     ~Call() {
@@ -131,13 +131,13 @@ struct Call {
         return st;
     }
     // n zeroed T.  Between begin_zeros() and end_zeros() the buffers are
-    // only carved, and end_zeros() clears them with one fill when they lie
-    // in one chunk (carved back to back), else one fill each.
+    // only carved, and end_zeros() clears them with one fill per chunk they
+    // lie in (carved back to back there: the alignment gaps between them,
+    // this call's too, are cleared with them).
     template <class T> ef_status zeros(T *&p, size_t n) {
         EF_TRY(alloc(p, n));
         if (zero_run) {
-            pending_zeros.push_back({p, n * sizeof(T) + 16});
-            if (buffers.size() != zero_chunk) zero_split = true;
+            pending_zeros.push_back({reinterpret_cast<char *>(p), n * sizeof(T) + 16, buffers.size()});
             return EF_OK;
         }
         EF_HIP(hipMemsetAsync(p, 0, n * sizeof(T) + 16, stream));
@@ -145,20 +145,18 @@ struct Call {
     }
     void begin_zeros() {
         zero_run = true;
-        zero_split = false;
-        zero_chunk = buffers.size();
         pending_zeros.clear();
     }
     ef_status end_zeros() {
         zero_run = false;
-        if (pending_zeros.empty()) return EF_OK;
-        if (!zero_split && zero_chunk == buffers.size()) {
-            char *lo = static_cast<char *>(pending_zeros.front().first);
-            const auto &last = pending_zeros.back();
-            const size_t span = static_cast<char *>(last.first) + last.second - lo;
-            EF_HIP(hipMemsetAsync(lo, 0, span, stream));
-        } else {
-            for (auto &z : pending_zeros) EF_HIP(hipMemsetAsync(z.first, 0, z.second, stream));
+        size_t i = 0;
+        while (i < pending_zeros.size()) {
+            char *lo = pending_zeros[i].p, *hi = lo + pending_zeros[i].bytes;
+            size_t j = i + 1;
+            for (; j < pending_zeros.size() && pending_zeros[j].chunk == pending_zeros[i].chunk; j++)
+                hi = pending_zeros[j].p + pending_zeros[j].bytes;
+            EF_HIP(hipMemsetAsync(lo, 0, (size_t)(hi - lo), stream));
+            i = j;
         }
         pending_zeros.clear();
         return EF_OK;
