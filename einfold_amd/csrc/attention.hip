@@ -98,17 +98,31 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));   // one 16-byte chunk
 
-// One value into an operand array of `plane` elements per plane (BF16x3
-// writes hi and lo planes).
+// Four consecutive values (idx a multiple of 4, the planes 16-byte
+// aligned) with one store per plane.
 template <int P, class T>
-__device__ __forceinline__ void put_split(typename Elem<P>::T *base, int64_t plane, int64_t idx, T x) {
+__device__ __forceinline__ void put_split4(typename Elem<P>::T *base, int64_t plane, int64_t idx, const T (&x)[4]) {
     if constexpr (P == PREC_BF16X3) {
-        __bf16 h, l;
-        split_bf16((float)x, h, l);
-        base[idx] = h;
-        base[plane + idx] = l;
+        bf16x4 h, l;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            __bf16 hj, lj;
+            split_bf16((float)x[j], hj, lj);
+            h[j] = hj;
+            l[j] = lj;
+        }
+        *reinterpret_cast<bf16x4 *>(base + idx) = h;
+        *reinterpret_cast<bf16x4 *>(base + plane + idx) = l;
+    } else if constexpr (P == PREC_BF16) {
+        bf16x4 h;
+#pragma unroll
+        for (int j = 0; j < 4; j++) h[j] = (__bf16)(float)x[j];
+        *reinterpret_cast<bf16x4 *>(base + idx) = h;
+    } else if constexpr (P == PREC_F32) {
+        *reinterpret_cast<f32x4 *>(base + idx) = f32x4{(float)x[0], (float)x[1], (float)x[2], (float)x[3]};
     } else {
-        base[idx] = (typename Elem<P>::T)x;
+#pragma unroll
+        for (int j = 0; j < 4; j++) base[idx + j] = (double)x[j];
     }
 }
 
@@ -173,18 +187,25 @@ __global__ __launch_bounds__(256) void prep_kernel(PrepArgs a) {
     E *rows = static_cast<E *>(j.rows), *trans = static_cast<E *>(j.trans);
     const int64_t T = j.T, Tp = j.Tp;
     const int64_t plane = a.B * Tp * D;
-    for (int i = threadIdx.x; i < 64 * D; i += 256) {
-        const int r = i / D, c = i % D;
+    // Four consecutive columns of a row per thread, then four consecutive
+    // rows of a column: a store of 8 or 16 bytes per plane instead of 2 or 4.
+    for (int i = threadIdx.x; i < 64 * D / 4; i += 256) {
+        const int r = i / (D / 4), c = (i % (D / 4)) * 4;
         const int64_t tt = t0 + r;
-        const In v = tt < T ? x[(b * T + tt) * D + c] : In(0);
-        t[r][c] = v;
-        if (rows) put_split<P>(rows, plane, (b * Tp + tt) * D + c, v);
+        In v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            v[j] = tt < T ? x[(b * T + tt) * D + c + j] : In(0);
+            t[r][c + j] = v[j];
+        }
+        if (rows) put_split4<P>(rows, plane, (b * Tp + tt) * D + c, v);
     }
     if (!trans) return;
     __syncthreads();
-    for (int i = threadIdx.x; i < 64 * D; i += 256) {
-        const int c = i / 64, r = i % 64;
-        put_split<P>(trans, plane, (b * D + c) * Tp + t0 + r, t[r][c]);
+    for (int i = threadIdx.x; i < 64 * D / 4; i += 256) {
+        const int c = i / 16, r = (i % 16) * 4;
+        const In v[4] = {t[r][c], t[r + 1][c], t[r + 2][c], t[r + 3][c]};
+        put_split4<P>(trans, plane, (b * D + c) * Tp + t0 + r, v);
     }
 }
 
