@@ -61,8 +61,13 @@ __global__ __launch_bounds__(256) void cast_keys_kernel(const T *__restrict__ in
             hi = shi[i] > hi ? shi[i] : hi;
             bad |= sbad[i];
         }
-        atomicMin((long long *)&stats->lo, (long long)lo);
-        atomicMax((long long *)&stats->hi, (long long)hi);
+        // A bound only moves one way, so a read of it (perhaps older, never
+        // tighter) that already covers this block's makes the atomic a
+        // no-op: skipped, the blocks' atomics on one line no longer queue.
+        if (lo < __hip_atomic_load(&stats->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin((long long *)&stats->lo, (long long)lo);
+        if (hi > __hip_atomic_load(&stats->hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax((long long *)&stats->hi, (long long)hi);
         if (bad) atomicOr(&stats->bad, 1);
     }
 }
