@@ -284,7 +284,9 @@ __global__ __launch_bounds__(256) void widen_keys_kernel(int64_t n, const T *__r
 
 ef_status cast_keys(Type t, const void *in, int64_t n, int64_t *out, KeyStats *stats, hipStream_t s) {
     if (n == 0) return EF_OK;
-    const unsigned g = blocks_for(n, 256) > 256 ? 256 : blocks_for(n, 256);   // at most 256 atomics per bound
+    // Up to 4096 blocks (eight waves per SIMD for large columns); a block's
+    // bound atomics run only when they move a bound.
+    const unsigned g = blocks_for(n, 256) > 4096 ? 4096 : blocks_for(n, 256);
 #define EF_CAST(T) hipLaunchKernelGGL(cast_keys_kernel<T>, dim3(g), dim3(256), 0, s, (const T *)in, n, out, stats)
     switch (t) {
     case Type::Int8: EF_CAST(int8_t); break;
