@@ -191,33 +191,56 @@ __global__ __launch_bounds__(256) void gustavson_kernel(GustavsonArgs g) {
         if (x >= g.nx) break;
         if (lane == 0) count[wv] = 0;
         wave_sync_lds();
-        for (uint32_t i = g.a_start[x]; i < g.a_start[x + 1]; i++) {
-            const uint32_t z = g.a_z[i];
-            const bool av = g.a_valid[i] != 0;
-            const double p = g.a_value[i];
-            const uint32_t bs = g.b_start[z], be = g.b_start[z + 1];
-            for (uint32_t j = bs + lane; j < be; j += 64) {
-                const uint32_t y = g.b_y[j];
-                if (j != bs && g.b_y[j - 1] == y) continue;   // not the head of its run
-                uint8_t st = state[y];
-                if (!(st & 1)) {
-                    st |= 1;
-                    const uint32_t t = atomicAdd(&count[wv], 1u);
-                    if (t < CAP) touched[wv][t] = y;
-                }
-                double a = acc[y];
-                for (uint32_t k = j; k < be && g.b_y[k] == y; k++) {
-                    if (av && g.b_valid[k]) {
-                        a = a + p * g.b_value[k];
+        // The rows of a, up to 64 at a time, one per lane with its bucket of
+        // b, broadcast row by row; a run head's row of b read with its run
+        // test.  The same products in the same order.
+        const uint32_t ia = g.a_start[x], ie = g.a_start[x + 1];
+        for (uint32_t r0 = ia; r0 < ie; r0 += 64) {
+            const uint32_t nr = ie - r0 < 64 ? ie - r0 : 64;
+            int rav = 0;
+            double rp = 0.0;
+            uint32_t rbs = 0, rbe = 0;
+            if ((uint32_t)lane < nr) {
+                const uint32_t z = g.a_z[r0 + lane];
+                rav = g.a_valid[r0 + lane] != 0;
+                rp = g.a_value[r0 + lane];
+                rbs = g.b_start[z];
+                rbe = g.b_start[z + 1];
+            }
+            for (uint32_t r = 0; r < nr; r++) {
+                const bool av = __shfl(rav, (int)r) != 0;
+                const double p = __shfl(rp, (int)r);
+                const uint32_t bs = (uint32_t)__shfl((int)rbs, (int)r), be = (uint32_t)__shfl((int)rbe, (int)r);
+                for (uint32_t j = bs + lane; j < be; j += 64) {
+                    const uint32_t y = g.b_y[j];
+                    const uint32_t yp = g.b_y[j != bs ? j - 1 : j];
+                    const uint8_t bv = g.b_valid[j];
+                    const double bx = g.b_value[j];
+                    if (j != bs && yp == y) continue;   // not the head of its run
+                    uint8_t st = state[y];
+                    if (!(st & 1)) {
+                        st |= 1;
+                        const uint32_t t = atomicAdd(&count[wv], 1u);
+                        if (t < CAP) touched[wv][t] = y;
+                    }
+                    double a = acc[y];
+                    if (av && bv) {
+                        a = a + p * bx;
                         st |= 2;
                     }
+                    for (uint32_t k = j + 1; k < be && g.b_y[k] == y; k++) {
+                        if (av && g.b_valid[k]) {
+                            a = a + p * g.b_value[k];
+                            st |= 2;
+                        }
+                    }
+                    acc[y] = a;
+                    state[y] = st;
                 }
-                acc[y] = a;
-                state[y] = st;
+                // This row's sums land before the next row's lanes read them.
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
             }
-            // This row's sums land before the next row's lanes read them.
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
         }
         wave_sync_lds();
         const uint32_t nt = count[wv];

@@ -281,6 +281,22 @@ def test_discriminate_matches_oracle_bits(cuda, seed, strict):
     assert_same(got.batches, want)
 
 
+def test_discriminate_many_output_columns_matches_oracle_bits(cuda):
+    """More than 512 output columns and no heavy join rank (every bucket of
+    b below 1024 rows): the Gustavson kernel with touched-column lists,
+    groups and sums bit-identical to the oracle's (discriminate.rs:548-583)."""
+    rng = np.random.default_rng(21)
+    n = 3000
+    a = pa.table({"i": pa.array(rng.integers(0, 50, n)), "j": pa.array(rng.integers(0, 40, n)),
+                  "v": pa.array(rng.normal(size=n))})
+    b = pa.table({"j": pa.array(rng.integers(0, 40, n)), "k": pa.array(rng.integers(0, 2000, n)),
+                  "v": pa.array(rng.normal(size=n), mask=rng.random(n) < 0.05)})
+    got, algo, want = _both_fold(_grid_fold(), a, b)
+    assert algo == got.algorithm == "discriminate"
+    assert pa.Table.from_batches(want).num_rows > 512
+    assert_same(got.batches, want)
+
+
 def test_discriminate_hash_keys_deterministic_across_batching(cuda):
     """The sparse product's keys: UInt64 hashes with a Zipf join key; bits
     equal to the oracle's and the same at any batch size."""
